@@ -1,0 +1,189 @@
+/*
+ * msosm_c.h -- C ABI of the MI355X-native MS-OSM coherent-dedispersion
+ * engine (libmsosm.so).
+ *
+ * Plain C types only: opaque handles, pointers, sizes.  Device pointers are
+ * HIP device addresses passed as void*; a HIP stream is passed as void*
+ * (NULL = the engine's own stream).  Complex samples are interleaved float
+ * pairs {re, im} (layout-compatible with the reference's `Complex`, a
+ * cufftComplex, utils/include/globals.h:28) and input samples are packed
+ * offset-binary uint16 pairs (`uint16_pair`, globals.h:21-25).
+ *
+ * Every function returns MSOSM_OK (0) or an error code; msosm_last_error()
+ * gives the message of the last failure on the calling thread.  The C++
+ * facade in include/msosm/ turns a failure into the reference's
+ * print-and-exit(1) behaviour (gpu/include/gpu_common.cuh:41-61).
+ *
+ * Each entry point names the reference interface it replaces.
+ */
+#ifndef MSOSM_C_H
+#define MSOSM_C_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes -------------------------------------------------------- */
+#define MSOSM_OK 0
+#define MSOSM_ERR_ARG 1         /* invalid argument                          */
+#define MSOSM_ERR_HIP 2         /* a HIP runtime call failed                 */
+#define MSOSM_ERR_STATE 3       /* call out of order                         */
+#define MSOSM_ERR_UNSUPPORTED 4 /* valid request this build does not support */
+#define MSOSM_ERR_ALLOC 5       /* device or host allocation failed          */
+
+/* ---- plan descriptor ------------------------------------------------------ */
+#define MSOSM_METHOD_MSOSM 0 /* multi-segment overlap-save (msosm_prepare.cpp) */
+#define MSOSM_METHOD_OSM 1   /* single-segment overlap-save (osm_prepare.cpp) */
+
+#define MSOSM_CHIRP_REF_F32W 0  /* chirp on the reference's float-rounded grid */
+#define MSOSM_CHIRP_EXACT_F64 1 /* chirp phase on an exact FP64 grid           */
+
+/* Execution forms (all produce the same output layout [dm][count][M]):      */
+#define MSOSM_FORM_PLAIN_F32 0 /* reference structure: per-DM product ring,   */
+                               /* separate inverse FFT and discard (FP32)     */
+#define MSOSM_FORM_FUSED_F32 1 /* shared spectra ring, gather*H fused into    */
+                               /* the inverse FFT, discard fused (FP32)       */
+#define MSOSM_FORM_FUSED_F16 2 /* as FUSED_F32 with FP16 spectra ring and H   */
+                               /* (pre-scaled), FP32 arithmetic               */
+
+typedef struct msosm_plan_desc {
+    int num_dms;             /* DM trials sharing one input stream (>=1)      */
+    const float *bw;         /* [num_dms] complex bandwidth = sample rate, Hz */
+    const float *dm;         /* [num_dms] dispersion measure, pc cm^-3 (>=0)  */
+    const float *f0;         /* [num_dms] lowest frequency of the band, Hz    */
+    unsigned long fftpoint;  /* requested FFT size 2M, 0 = minimum            */
+    int count;               /* blocks of M samples per call (>=1)            */
+    unsigned long input_size;/* !=0: count = input_size / M                   */
+    int method;              /* MSOSM_METHOD_*                                */
+    int chirp;               /* MSOSM_CHIRP_*                                 */
+    int form;                /* MSOSM_FORM_*                                  */
+} msosm_plan_desc;
+
+typedef struct msosm_plan_info {
+    unsigned long M;         /* hop (segment length); FFT size is 2M          */
+    unsigned long M_min;     /* minimum M over the DMs                        */
+    unsigned long fftpoint;  /* 2M                                            */
+    int count;               /* blocks per call                               */
+    int num_dms;
+    int K_max;               /* largest segment count (delaycount)            */
+    int R;                   /* ring slots of the reference layout            */
+                             /* (max delay_block_size)                         */
+    int clamped;             /* 1 if fftpoint was raised to the minimum       */
+} msosm_plan_info;
+
+typedef struct msosm_plan msosm_plan;
+typedef struct msosm_engine msosm_engine;
+typedef struct msosm_fold msosm_fold;
+
+/* Planner (host only; no GPU needed).
+ * Replaces Prepare_MSOSM / Prepare_OSM / Prepare_MSOSM_DM_concurrent ctor +
+ * calculate_min_order + segmentation (dedispersion/msosm/msosm_prepare.cpp:3-152,
+ * msosm_dm_concurrent/msosm_prepare_dm_concurrent.cpp:3-169,
+ * overlap_save/osm_prepare.cpp:3-30) and the fftpoint/count logic of
+ * initialize_uint16 (msosm_gpu_dm_concurrent.cpp:28-42). */
+int msosm_plan_create(const msosm_plan_desc *desc, msosm_plan **out);
+void msosm_plan_destroy(msosm_plan *plan);
+int msosm_plan_get_info(const msosm_plan *plan, msosm_plan_info *info);
+/* Per-DM scalars: Nd, M_min, delaycount K, delay_block_size. */
+int msosm_plan_dm_info(const msosm_plan *plan, int dm_index, unsigned long *nd, unsigned long *m_min,
+                       int *delaycount, int *delay_block_size);
+/* Per-bin delay table delay_points[2M] (Prepare_MSOSM::segmentation,
+ * msosm_prepare.cpp:133-147). */
+int msosm_plan_delay_points(const msosm_plan *plan, int dm_index, int32_t *out);
+
+/* Engine (GPU).
+ * Replaces MSOSM_GPU_BATCH / MSOSM_GPU_DM_concurrent / OSM_GPU_BATCH /
+ * OSM_GPU_DM_concurrent ::initialize_uint16 device part
+ * (msosm_gpu_dm_concurrent.cpp:49-108, msosm_gpu_batch.cpp:32-72). */
+int msosm_engine_create(const msosm_plan *plan, void *hip_stream, msosm_engine **out);
+int msosm_engine_destroy(msosm_engine *eng);
+/* One call: count*M uint16 pairs; where = 0 host pointer, 1 device pointer.
+ * Replaces filter_block_uint16 (msosm_gpu_dm_concurrent.cpp:119-151,
+ * msosm_gpu_batch.cpp:75-94, osm_gpu_batch.cpp:94-123). */
+int msosm_engine_filter_u16(msosm_engine *eng, const void *input, int where);
+/* Output of the last call, layout [dm][count][M] complex64.
+ * Replaces get_output(Complex*) (msosm_gpu_dm_concurrent.cpp:153-165). */
+int msosm_engine_output_c64(msosm_engine *eng, void *dst, int where);
+/* Same, packed to uint16 pairs (complexToUint16, gpu_kernel.cu:317-329);
+ * replaces get_output(uint16_pair*) (msosm_gpu_dm_concurrent.cpp:167-180). */
+int msosm_engine_output_u16(msosm_engine *eng, void *dst, int where);
+/* Device address of the complex64 output buffer (the reference's public
+ * output_buffer_d, msosm_gpu_batch.h:28).  It holds num_dms*count*M + 1
+ * elements; the extra element is the reference's sentinel (always 0). */
+int msosm_engine_output_ptr(msosm_engine *eng, void **dev_ptr);
+/* Filter table H of one DM (complex64, 2M), copied to host (tests). */
+int msosm_engine_dedisp_params(msosm_engine *eng, int dm_index, float *host_out);
+/* Block until the engine's stream is idle (synchronize()). */
+int msosm_engine_synchronize(msosm_engine *eng);
+/* Zero the stream history (overlap tail, delay ring, block counter). */
+int msosm_engine_reset(msosm_engine *eng);
+/* The HIP stream the engine launches on (void* = hipStream_t). */
+int msosm_engine_stream(msosm_engine *eng, void **hip_stream);
+/* Per-stage timing with HIP events on the engine stream (0 = off).
+ * Stage ids: MSOSM_STAGE_*.  times_ms[i] = summed kernel time of stage i,
+ * calls[i] = launches timed. */
+#define MSOSM_STAGE_UNPACK 0
+#define MSOSM_STAGE_FFT_FWD 1
+#define MSOSM_STAGE_MULTIPLY 2
+#define MSOSM_STAGE_FFT_INV 3
+#define MSOSM_STAGE_DISCARD 4
+#define MSOSM_STAGE_COPY 5
+#define MSOSM_NUM_STAGES 6
+int msosm_engine_set_timing(msosm_engine *eng, int on);
+int msosm_engine_get_timing(msosm_engine *eng, double *times_ms, long long *calls, int n);
+
+/* Element kernels on device pointers (gpu/include/gpu_kernel.cuh:5-27).
+ * stream NULL = default stream. */
+int msosm_k_uint16_to_complex(float *dst, const void *src, size_t n, void *stream);    /* :9  */
+int msosm_k_complex_to_uint16(void *dst, const float *src, size_t n, void *stream);    /* :10 */
+int msosm_k_intensity(const float *a, const float *b, float *out, size_t n, void *stream); /* :11 */
+/* ComplexMultiply_concurrent_batch (:22; numDMs=1 is ComplexMultiply_batch). */
+int msosm_k_multiply_batch(const float *a, const float *b, int N, float *out, int count, int num_dms,
+                           void *stream);
+/* ComplexMultiplyDelay_concurrent_batch (:23; numDMs=1 is the batch form). */
+int msosm_k_multiply_delay_batch(const float *a, const float *b, int N, float *ring, const int32_t *delay,
+                                 int delay_block_size, int index, int count, int num_dms, void *stream);
+/* DiscardSamples_concurrent_batch (:24; numDMs=1 is the batch form). */
+int msosm_k_discard_batch(const float *a, float *b, int M, int count, int num_dms, void *stream);
+/* Batched C2C FFT of power-of-two size n (the build's replacement for the
+ * cufftPlanMany/cufftExecC2C call sites, e.g. msosm_gpu_batch.cpp:41-49,84,91).
+ * Transform b reads in[b*idist + k], writes out[b*odist + k]; direction -1
+ * forward, +1 inverse (unnormalised).  in and out must not overlap. */
+int msosm_k_fft_c2c(const float *in, float *out, size_t n, size_t batch, size_t idist, size_t odist,
+                    int direction, void *stream);
+
+/* Fold (folding/include/fold_gpu.h:10-41).  The fold runs on the GPU; bin
+ * assignment follows Fold_GPU::fold_data exactly (fold_gpu.cpp:48-173). */
+int msosm_fold_create(float period, float fs, unsigned long size, unsigned long time_bin, void *hip_stream,
+                      msosm_fold **out);                                     /* ctor :3-27 */
+int msosm_fold_destroy(msosm_fold *f);
+int msosm_fold_discard_samples(msosm_fold *f, unsigned long nd);             /* :29-33 */
+/* Intensity of two polarisations' device outputs (size complex samples each);
+ * pol2 may be NULL (single polarisation).  calculate_intensity, :35-43. */
+int msosm_fold_intensity(msosm_fold *f, const void *pol1_dev, const void *pol2_dev);
+/* Fold the current intensity buffer (fold_data, :48-173). */
+int msosm_fold_fold_data(msosm_fold *f);
+/* Average raw bins into time_bin bins (fold_data_bins, :175-194). */
+int msosm_fold_bins(msosm_fold *f);
+/* Copy results to host: folded[time_bin], raw[period_samples],
+ * intensity[size]; any pointer may be NULL. */
+int msosm_fold_get(msosm_fold *f, float *folded, float *raw, float *intensity);
+int msosm_fold_info(msosm_fold *f, unsigned long *period_samples, unsigned long *time_bin, int *ready);
+/* Device address of the intensity buffer (size floats) for callers that
+ * produce intensities themselves. */
+int msosm_fold_intensity_ptr(msosm_fold *f, void **dev_ptr);
+/* Load size intensities from host (where = 0) or device (where = 1) memory
+ * into the fold's buffer (the reference's public total_intensity array,
+ * fold_gpu.h:31, which fold_data reads). */
+int msosm_fold_load_intensity(msosm_fold *f, const float *src, int where);
+
+const char *msosm_last_error(void);
+const char *msosm_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

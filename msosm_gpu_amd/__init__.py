@@ -1,0 +1,375 @@
+"""Python binding of libmsosm.so, the MI355X-native MS-OSM coherent
+dedispersion engine (C ABI: include/msosm_c.h).
+
+The library is built in-tree (msosm_gpu_amd/lib/libmsosm.so) by
+msosm_gpu_amd.build.  There is no fallback: every call goes to the HIP
+library, and a missing library raises.
+
+Device buffers may be passed as torch CUDA tensors (data_ptr) or as raw
+integer device addresses; host buffers as numpy arrays.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libmsosm.so")
+INCLUDE_DIR = os.path.join(os.path.dirname(HERE), "include")
+
+OK = 0
+METHOD_MSOSM, METHOD_OSM = 0, 1
+CHIRP_REF_F32W, CHIRP_EXACT_F64 = 0, 1
+FORM_PLAIN_F32, FORM_FUSED_F32, FORM_FUSED_F16 = 0, 1, 2
+STAGES = ("unpack", "fft_fwd", "multiply", "fft_inv", "discard", "copy")
+
+_lib = None
+
+
+class MsosmError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"msosm error {code}: {msg}")
+        self.code = code
+
+
+class PlanDesc(ctypes.Structure):
+    _fields_ = [
+        ("num_dms", ctypes.c_int),
+        ("bw", ctypes.POINTER(ctypes.c_float)),
+        ("dm", ctypes.POINTER(ctypes.c_float)),
+        ("f0", ctypes.POINTER(ctypes.c_float)),
+        ("fftpoint", ctypes.c_ulong),
+        ("count", ctypes.c_int),
+        ("input_size", ctypes.c_ulong),
+        ("method", ctypes.c_int),
+        ("chirp", ctypes.c_int),
+        ("form", ctypes.c_int),
+    ]
+
+
+class PlanInfo(ctypes.Structure):
+    _fields_ = [
+        ("M", ctypes.c_ulong),
+        ("M_min", ctypes.c_ulong),
+        ("fftpoint", ctypes.c_ulong),
+        ("count", ctypes.c_int),
+        ("num_dms", ctypes.c_int),
+        ("K_max", ctypes.c_int),
+        ("R", ctypes.c_int),
+        ("clamped", ctypes.c_int),
+    ]
+
+
+# Exported C ABI: name -> (restype, argtypes)
+_vp, _i, _ul, _sz = ctypes.c_void_p, ctypes.c_int, ctypes.c_ulong, ctypes.c_size_t
+_f, _ll = ctypes.c_float, ctypes.c_longlong
+SIGNATURES = {
+    "msosm_plan_create": (_i, [ctypes.POINTER(PlanDesc), ctypes.POINTER(_vp)]),
+    "msosm_plan_destroy": (None, [_vp]),
+    "msosm_plan_get_info": (_i, [_vp, ctypes.POINTER(PlanInfo)]),
+    "msosm_plan_dm_info": (_i, [_vp, _i, ctypes.POINTER(_ul), ctypes.POINTER(_ul), ctypes.POINTER(_i),
+                                ctypes.POINTER(_i)]),
+    "msosm_plan_delay_points": (_i, [_vp, _i, _vp]),
+    "msosm_engine_create": (_i, [_vp, _vp, ctypes.POINTER(_vp)]),
+    "msosm_engine_destroy": (_i, [_vp]),
+    "msosm_engine_filter_u16": (_i, [_vp, _vp, _i]),
+    "msosm_engine_output_c64": (_i, [_vp, _vp, _i]),
+    "msosm_engine_output_u16": (_i, [_vp, _vp, _i]),
+    "msosm_engine_output_ptr": (_i, [_vp, ctypes.POINTER(_vp)]),
+    "msosm_engine_dedisp_params": (_i, [_vp, _i, _vp]),
+    "msosm_engine_synchronize": (_i, [_vp]),
+    "msosm_engine_reset": (_i, [_vp]),
+    "msosm_engine_stream": (_i, [_vp, ctypes.POINTER(_vp)]),
+    "msosm_engine_set_timing": (_i, [_vp, _i]),
+    "msosm_engine_get_timing": (_i, [_vp, _vp, _vp, _i]),
+    "msosm_k_uint16_to_complex": (_i, [_vp, _vp, _sz, _vp]),
+    "msosm_k_complex_to_uint16": (_i, [_vp, _vp, _sz, _vp]),
+    "msosm_k_intensity": (_i, [_vp, _vp, _vp, _sz, _vp]),
+    "msosm_k_multiply_batch": (_i, [_vp, _vp, _i, _vp, _i, _i, _vp]),
+    "msosm_k_multiply_delay_batch": (_i, [_vp, _vp, _i, _vp, _vp, _i, _i, _i, _i, _vp]),
+    "msosm_k_discard_batch": (_i, [_vp, _vp, _i, _i, _i, _vp]),
+    "msosm_k_fft_c2c": (_i, [_vp, _vp, _sz, _sz, _sz, _sz, _i, _vp]),
+    "msosm_fold_create": (_i, [_f, _f, _ul, _ul, _vp, ctypes.POINTER(_vp)]),
+    "msosm_fold_destroy": (_i, [_vp]),
+    "msosm_fold_discard_samples": (_i, [_vp, _ul]),
+    "msosm_fold_intensity": (_i, [_vp, _vp, _vp]),
+    "msosm_fold_fold_data": (_i, [_vp]),
+    "msosm_fold_bins": (_i, [_vp]),
+    "msosm_fold_get": (_i, [_vp, _vp, _vp, _vp]),
+    "msosm_fold_info": (_i, [_vp, ctypes.POINTER(_ul), ctypes.POINTER(_ul), ctypes.POINTER(_i)]),
+    "msosm_fold_intensity_ptr": (_i, [_vp, ctypes.POINTER(_vp)]),
+    "msosm_fold_load_intensity": (_i, [_vp, _vp, _i]),
+    "msosm_last_error": (ctypes.c_char_p, []),
+    "msosm_version": (ctypes.c_char_p, []),
+}
+
+
+def lib():
+    """Load libmsosm.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m msosm_gpu_amd.build` "
+                           "(there is no CPU fallback)")
+    L = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def check(rc):
+    if rc != OK:
+        raise MsosmError(rc, lib().msosm_last_error().decode())
+
+
+def _ptr(x):
+    """Device/host address of a torch tensor, numpy array or int."""
+    if x is None:
+        return None
+    if isinstance(x, int):
+        return x
+    if isinstance(x, np.ndarray):
+        return x.ctypes.data
+    if hasattr(x, "data_ptr"):
+        return x.data_ptr()
+    raise TypeError(f"cannot take the address of {type(x)}")
+
+
+def _is_device(x):
+    return hasattr(x, "is_cuda") and x.is_cuda
+
+
+def _stream_handle(stream):
+    if stream is None:
+        return None
+    if isinstance(stream, int):
+        return stream
+    if hasattr(stream, "cuda_stream"):
+        return stream.cuda_stream
+    raise TypeError("stream must be None, an int handle or a torch.cuda.Stream")
+
+
+def _farr(v, n):
+    a = np.broadcast_to(np.asarray(v, np.float32), (n,)).copy()
+    return a
+
+
+class Plan:
+    """Planner (host only): MS-OSM / OSM plan for num_dms DM trials."""
+
+    def __init__(self, bw, dm, f0, fftpoint=0, count=1, input_size=0, method=METHOD_MSOSM, chirp=CHIRP_REF_F32W,
+                 form=FORM_PLAIN_F32):
+        n = max(np.size(bw), np.size(dm), np.size(f0))
+        self._bw, self._dm, self._f0 = _farr(bw, n), _farr(dm, n), _farr(f0, n)
+        fp = ctypes.POINTER(ctypes.c_float)
+        d = PlanDesc(n, self._bw.ctypes.data_as(fp), self._dm.ctypes.data_as(fp), self._f0.ctypes.data_as(fp),
+                     fftpoint, count, input_size, method, chirp, form)
+        h = ctypes.c_void_p()
+        check(lib().msosm_plan_create(ctypes.byref(d), ctypes.byref(h)))
+        self.h = h
+        info = PlanInfo()
+        check(lib().msosm_plan_get_info(self.h, ctypes.byref(info)))
+        self.M, self.M_min, self.fftpoint = int(info.M), int(info.M_min), int(info.fftpoint)
+        self.count, self.num_dms, self.K_max = int(info.count), int(info.num_dms), int(info.K_max)
+        self.R, self.clamped = int(info.R), bool(info.clamped)
+        self.method, self.chirp, self.form = method, chirp, form
+        self.bw, self.dm, self.f0 = self._bw, self._dm, self._f0
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.msosm_plan_destroy(self.h)
+            self.h = None
+
+    def dm_info(self, i):
+        nd, mm = ctypes.c_ulong(), ctypes.c_ulong()
+        K, dbs = ctypes.c_int(), ctypes.c_int()
+        check(lib().msosm_plan_dm_info(self.h, i, ctypes.byref(nd), ctypes.byref(mm), ctypes.byref(K),
+                                       ctypes.byref(dbs)))
+        return dict(Nd=int(nd.value), M_min=int(mm.value), K=int(K.value), delay_block_size=int(dbs.value))
+
+    def delay_points(self, i):
+        out = np.zeros(self.fftpoint, np.int32)
+        check(lib().msosm_plan_delay_points(self.h, i, out.ctypes.data))
+        return out
+
+
+class Engine:
+    """GPU engine: filter count*M samples per call for all DM trials."""
+
+    def __init__(self, plan, stream=None):
+        self.plan = plan
+        h = ctypes.c_void_p()
+        check(lib().msosm_engine_create(plan.h, _stream_handle(stream), ctypes.byref(h)))
+        self.h = h
+        s = ctypes.c_void_p()
+        check(lib().msosm_engine_stream(self.h, ctypes.byref(s)))
+        self.stream = s.value
+
+    def close(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.msosm_engine_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    @property
+    def out_shape(self):
+        p = self.plan
+        return (p.num_dms, p.count, p.M)
+
+    def filter_u16(self, x):
+        """x: count*M packed uint16 pairs: numpy (host) or torch CUDA tensor."""
+        n = self.plan.count * self.plan.M
+        if isinstance(x, np.ndarray):
+            x = np.ascontiguousarray(x)
+            if x.nbytes != 4 * n:
+                raise ValueError(f"input must hold {n} uint16 pairs ({4 * n} bytes), got {x.nbytes}")
+            check(lib().msosm_engine_filter_u16(self.h, x.ctypes.data, 0))
+        else:
+            if x.numel() * x.element_size() != 4 * n:
+                raise ValueError(f"input must hold {n} uint16 pairs")
+            check(lib().msosm_engine_filter_u16(self.h, _ptr(x), 1 if _is_device(x) else 0))
+
+    def output_c64(self, out=None):
+        if out is None:
+            out = np.empty(self.out_shape, np.complex64)
+            check(lib().msosm_engine_output_c64(self.h, out.ctypes.data, 0))
+            return out
+        check(lib().msosm_engine_output_c64(self.h, _ptr(out), 1 if _is_device(out) else 0))
+        return out
+
+    def output_u16(self, out=None):
+        if out is None:
+            out = np.empty(self.out_shape + (2,), np.uint16)
+            check(lib().msosm_engine_output_u16(self.h, out.ctypes.data, 0))
+            return out
+        check(lib().msosm_engine_output_u16(self.h, _ptr(out), 1 if _is_device(out) else 0))
+        return out
+
+    def output_ptr(self):
+        p = ctypes.c_void_p()
+        check(lib().msosm_engine_output_ptr(self.h, ctypes.byref(p)))
+        return p.value
+
+    def dedisp_params(self, i):
+        out = np.empty(self.plan.fftpoint, np.complex64)
+        check(lib().msosm_engine_dedisp_params(self.h, i, out.ctypes.data))
+        return out
+
+    def synchronize(self):
+        check(lib().msosm_engine_synchronize(self.h))
+
+    def reset(self):
+        check(lib().msosm_engine_reset(self.h))
+
+    def set_timing(self, on=True):
+        check(lib().msosm_engine_set_timing(self.h, 1 if on else 0))
+
+    def get_timing(self):
+        ms = np.zeros(len(STAGES), np.float64)
+        calls = np.zeros(len(STAGES), np.int64)
+        check(lib().msosm_engine_get_timing(self.h, ms.ctypes.data, calls.ctypes.data, len(STAGES)))
+        return {name: (float(ms[i]), int(calls[i])) for i, name in enumerate(STAGES)}
+
+
+class Fold:
+    """GPU fold with the reference's Fold_GPU bin assignment."""
+
+    def __init__(self, period, fs, size, time_bin=-1, stream=None):
+        h = ctypes.c_void_p()
+        check(lib().msosm_fold_create(period, fs, size, ctypes.c_ulong(time_bin).value, _stream_handle(stream),
+                                      ctypes.byref(h)))
+        self.h = h
+        self.size = size
+        P, T, r = ctypes.c_ulong(), ctypes.c_ulong(), ctypes.c_int()
+        check(lib().msosm_fold_info(self.h, ctypes.byref(P), ctypes.byref(T), ctypes.byref(r)))
+        self.period_samples, self.time_bin = int(P.value), int(T.value)
+
+    def close(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.msosm_fold_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def discard_samples(self, nd):
+        check(lib().msosm_fold_discard_samples(self.h, nd))
+
+    def intensity(self, pol1, pol2=None):
+        check(lib().msosm_fold_intensity(self.h, _ptr(pol1), _ptr(pol2)))
+
+    def intensity_ptr(self):
+        p = ctypes.c_void_p()
+        check(lib().msosm_fold_intensity_ptr(self.h, ctypes.byref(p)))
+        return p.value
+
+    def load_intensity(self, x):
+        """x: size float32 values, numpy (host) or torch CUDA tensor."""
+        if isinstance(x, np.ndarray):
+            x = np.ascontiguousarray(x, np.float32)
+            assert x.size == self.size
+            check(lib().msosm_fold_load_intensity(self.h, x.ctypes.data, 0))
+        else:
+            assert x.numel() == self.size
+            check(lib().msosm_fold_load_intensity(self.h, _ptr(x), 1 if _is_device(x) else 0))
+
+    def fold_data(self):
+        check(lib().msosm_fold_fold_data(self.h))
+
+    def bins(self):
+        check(lib().msosm_fold_bins(self.h))
+
+    @property
+    def ready(self):
+        r = ctypes.c_int()
+        check(lib().msosm_fold_info(self.h, None, None, ctypes.byref(r)))
+        return bool(r.value)
+
+    def get(self):
+        folded = np.empty(self.time_bin, np.float32)
+        raw = np.empty(self.period_samples, np.float32)
+        inten = np.empty(self.size, np.float32)
+        check(lib().msosm_fold_get(self.h, folded.ctypes.data, raw.ctypes.data, inten.ctypes.data))
+        return folded, raw, inten
+
+
+# ---- element kernels on device buffers (torch tensors or addresses) -----------
+def uint16_to_complex(dst, src, n, stream=None):
+    check(lib().msosm_k_uint16_to_complex(_ptr(dst), _ptr(src), n, _stream_handle(stream)))
+
+
+def complex_to_uint16(dst, src, n, stream=None):
+    check(lib().msosm_k_complex_to_uint16(_ptr(dst), _ptr(src), n, _stream_handle(stream)))
+
+
+def intensity(a, b, out, n, stream=None):
+    check(lib().msosm_k_intensity(_ptr(a), _ptr(b), _ptr(out), n, _stream_handle(stream)))
+
+
+def multiply_batch(a, b, N, out, count, num_dms=1, stream=None):
+    check(lib().msosm_k_multiply_batch(_ptr(a), _ptr(b), N, _ptr(out), count, num_dms, _stream_handle(stream)))
+
+
+def multiply_delay_batch(a, b, N, ring, delay, delay_block_size, index, count, num_dms=1, stream=None):
+    check(lib().msosm_k_multiply_delay_batch(_ptr(a), _ptr(b), N, _ptr(ring), _ptr(delay), delay_block_size, index,
+                                             count, num_dms, _stream_handle(stream)))
+
+
+def discard_batch(a, b, M, count, num_dms=1, stream=None):
+    check(lib().msosm_k_discard_batch(_ptr(a), _ptr(b), M, count, num_dms, _stream_handle(stream)))
+
+
+def fft_c2c(inp, out, n, batch, idist=None, odist=None, direction=-1, stream=None):
+    idist = n if idist is None else idist
+    odist = n if odist is None else odist
+    check(lib().msosm_k_fft_c2c(_ptr(inp), _ptr(out), n, batch, idist, odist, direction, _stream_handle(stream)))
+
+
+def version():
+    return lib().msosm_version().decode()

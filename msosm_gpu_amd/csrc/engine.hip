@@ -1,0 +1,688 @@
+// engine.hip -- the dedispersion engine behind the C ABI (include/msosm_c.h).
+//
+// One engine = one input stream of packed uint16 complex samples, num_dms DM
+// trials sharing one FFT size 2M, count blocks of M samples per call.
+//
+// PLAIN_F32 keeps the reference's structure (msosm_gpu_dm_concurrent.cpp:
+// 119-151): unpack into an (count+1)*M buffer whose head holds the previous
+// call's tail, forward FFT of count overlapped windows, multiply by H and
+// scatter into a per-DM product ring [R/count][dm][count][2M], inverse FFT of
+// the ring group at the cursor, discard.  FUSED_F32 / FUSED_F16 store each
+// forward spectrum once in a ring shared by all DMs and fold the per-bin delay
+// gather, the H multiply, the discard and the scale into the inverse-FFT
+// kernel.  All forms give output [dm][count][M] complex64 for block g equal to
+// the last M points of IDFT(X_{g-d(n)}[n] H[n]) / (2M)^2.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "common.h"
+#include "kernels.h"
+#include "msosm_c.h"
+#include "plan.h"
+
+namespace {
+thread_local std::string g_last_error;
+} // namespace
+
+int msosm::set_error(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+namespace {
+using msosm::set_error;
+
+#define HIP_TRY(expr)                                                                                            \
+    do {                                                                                                         \
+        hipError_t e__ = (expr);                                                                                 \
+        if (e__ != hipSuccess)                                                                                   \
+            return set_error(MSOSM_ERR_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e__), __FILE__,     \
+                             __LINE__);                                                                          \
+    } while (0)
+
+int ilog2_exact(unsigned long v)
+{
+    int l = 0;
+    while ((1ul << l) < v)
+        l++;
+    return l;
+}
+
+template <class T>
+hipError_t dev_alloc(T **p, size_t elems)
+{
+    *p = nullptr;
+    if (elems == 0)
+        elems = 1;
+    return hipMalloc((void **)p, elems * sizeof(T));
+}
+
+} // namespace
+
+struct msosm_plan {
+    msosm::Plan p;
+};
+
+struct StageTimer {
+    int stage;
+    hipEvent_t a, b;
+};
+
+struct msosm_engine {
+    msosm::Plan plan;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    int log2n = 0;
+    int form = 0;
+    // shared buffers
+    uint32_t *in_stage = nullptr;   // count*M packed input (host input path)
+    float2 *H32 = nullptr;          // [ndm][2M]
+    __half2 *H16 = nullptr;         // [ndm][2M] = H/M (FUSED_F16)
+    int32_t *delay32 = nullptr;     // [ndm][2M] (PLAIN)
+    uint16_t *delay16 = nullptr;    // [ndm][2M] (FUSED)
+    float2 *out = nullptr;          // [ndm][count][M] + 1 sentinel
+    uint32_t *out_u16 = nullptr;    // packed output + 1 sentinel
+    // PLAIN_F32
+    float2 *inbuf = nullptr;        // (count+1)*M
+    float2 *fftblk = nullptr;       // count*2M
+    float2 *ring = nullptr;         // ndm*R*2M
+    float2 *ifftblk = nullptr;      // ndm*count*2M
+    float2 *fft_scratch = nullptr;  // two-pass FFT scratch
+    int current_index = 0;
+    // FUSED
+    uint32_t *tail = nullptr;       // previous call's last M input samples
+    void *xring = nullptr;          // rx*2M spectra (float2 or half2)
+    int rx = 0;
+    long long g0 = 0;               // global index of the next block
+    bool have_output = false;
+    // timing
+    bool timing = false;
+    std::vector<StageTimer> pending;
+    std::vector<hipEvent_t> pool;
+    double stage_ms[MSOSM_NUM_STAGES] = {0};
+    long long stage_calls[MSOSM_NUM_STAGES] = {0};
+};
+
+namespace {
+
+hipEvent_t pool_get(msosm_engine *e)
+{
+    if (!e->pool.empty()) {
+        hipEvent_t ev = e->pool.back();
+        e->pool.pop_back();
+        return ev;
+    }
+    hipEvent_t ev = nullptr;
+    if (hipEventCreate(&ev) != hipSuccess)
+        return nullptr;
+    return ev;
+}
+
+void harvest(msosm_engine *e)
+{
+    for (auto &t : e->pending) {
+        float ms = 0;
+        if (hipEventSynchronize(t.b) == hipSuccess && hipEventElapsedTime(&ms, t.a, t.b) == hipSuccess) {
+            e->stage_ms[t.stage] += ms;
+            e->stage_calls[t.stage] += 1;
+        }
+        e->pool.push_back(t.a);
+        e->pool.push_back(t.b);
+    }
+    e->pending.clear();
+}
+
+struct StageScope {
+    msosm_engine *e;
+    int stage;
+    hipEvent_t a = nullptr;
+    StageScope(msosm_engine *eng, int st) : e(eng), stage(st)
+    {
+        if (e->timing) {
+            if (e->pending.size() > 4096)
+                harvest(e);
+            a = pool_get(e);
+            if (a)
+                hipEventRecord(a, e->stream);
+        }
+    }
+    ~StageScope()
+    {
+        if (e->timing && a) {
+            hipEvent_t b = pool_get(e);
+            if (b) {
+                hipEventRecord(b, e->stream);
+                e->pending.push_back({stage, a, b});
+            } else {
+                e->pool.push_back(a);
+            }
+        }
+    }
+};
+
+void free_engine(msosm_engine *e)
+{
+    if (!e)
+        return;
+    if (e->stream)
+        hipStreamSynchronize(e->stream);
+    harvest(e);
+    for (auto ev : e->pool)
+        hipEventDestroy(ev);
+    void *bufs[] = {e->in_stage, e->H32, e->H16, e->delay32, e->delay16, e->out, e->out_u16, e->inbuf, e->fftblk,
+                    e->ring, e->ifftblk, e->fft_scratch, e->tail, e->xring};
+    for (void *b : bufs)
+        if (b)
+            hipFree(b);
+    if (e->own_stream && e->stream)
+        hipStreamDestroy(e->stream);
+    delete e;
+}
+
+// Filter tables H = DFT_2M(zero-pad(IDFT_M(C))) for every DM, on the GPU
+// (msosm_prepare.cpp:37-84; the /M of :68-72 is overwritten by the memcpy at
+// :74, so the IDFT is unnormalised).
+int build_filter_tables(msosm_engine *e)
+{
+    const msosm::Plan &p = e->plan;
+    const unsigned long M = p.M, N = p.N;
+    const int ndm = p.num_dms;
+    std::vector<msosm::ChirpParams> prm(ndm);
+    for (int i = 0; i < ndm; i++) {
+        const msosm::DmModel &m = p.dms[i];
+        prm[i].fs = m.fs;
+        prm[i].step = (float)(2 * 3.14159265358979323846 * (double)m.fs / (double)M); // :42
+        prm[i].w0 = m.w0;
+        prm[i].kdm = m.kdm;
+        prm[i].dm = m.dm;
+        prm[i].f0 = m.f0;
+    }
+    msosm::ChirpParams *prm_d = nullptr;
+    float2 *c = nullptr, *h = nullptr, *pad = nullptr, *scratch = nullptr;
+    int rc = MSOSM_OK;
+    auto cleanup = [&]() {
+        hipStreamSynchronize(e->stream);
+        if (prm_d)
+            hipFree(prm_d);
+        if (c)
+            hipFree(c);
+        if (h)
+            hipFree(h);
+        if (pad)
+            hipFree(pad);
+        if (scratch)
+            hipFree(scratch);
+    };
+    auto fail = [&](hipError_t err, const char *what) {
+        cleanup();
+        return set_error(MSOSM_ERR_HIP, "filter tables: %s: %s", what, hipGetErrorString(err));
+    };
+    hipError_t err;
+    if ((err = dev_alloc(&prm_d, ndm)) != hipSuccess)
+        return fail(err, "alloc params");
+    if ((err = hipMemcpyAsync(prm_d, prm.data(), sizeof(msosm::ChirpParams) * ndm, hipMemcpyHostToDevice,
+                              e->stream)) != hipSuccess)
+        return fail(err, "copy params");
+    if ((err = dev_alloc(&c, (size_t)ndm * M)) != hipSuccess || (err = dev_alloc(&h, (size_t)ndm * M)) != hipSuccess ||
+        (err = dev_alloc(&pad, (size_t)ndm * N)) != hipSuccess)
+        return fail(err, "alloc tables");
+    size_t sc = std::max(msosm::fft_c2c_scratch_elems(ilog2_exact(M), ndm),
+                         msosm::fft_c2c_scratch_elems(ilog2_exact(N), ndm));
+    if (sc && (err = dev_alloc(&scratch, sc)) != hipSuccess)
+        return fail(err, "alloc scratch");
+    if ((err = msosm::launch_chirp(c, M, p.desc.chirp, prm_d, ndm, e->stream)) != hipSuccess)
+        return fail(err, "chirp");
+    if ((err = msosm::launch_fft_c2c(c, h, ilog2_exact(M), ndm, M, M, true, 1.0f, scratch, e->stream)) != hipSuccess)
+        return fail(err, "IDFT_M");
+    if ((err = msosm::launch_zero_pad(pad, h, M, ndm, e->stream)) != hipSuccess)
+        return fail(err, "pad");
+    if ((err = msosm::launch_fft_c2c(pad, e->H32, ilog2_exact(N), ndm, N, N, false, 1.0f, scratch, e->stream)) !=
+        hipSuccess)
+        return fail(err, "DFT_2M");
+    if (e->H16) {
+        if ((err = msosm::launch_to_half(e->H16, e->H32, (size_t)ndm * N, 1.0f / (float)M, e->stream)) != hipSuccess)
+            return fail(err, "H to FP16");
+    }
+    cleanup();
+    return rc;
+}
+
+int create_engine(const msosm::Plan &plan, void *stream, msosm_engine **out)
+{
+    msosm_engine *e = new msosm_engine();
+    e->plan = plan;
+    e->form = plan.desc.form;
+    e->log2n = ilog2_exact(plan.N);
+    const msosm::Plan &p = e->plan;
+    const unsigned long M = p.M, N = p.N;
+    const int ndm = p.num_dms, count = p.count;
+    auto bail = [&](int rc) {
+        free_engine(e);
+        return rc;
+    };
+    if (e->form != MSOSM_FORM_PLAIN_F32 && !msosm::fused_supported(e->log2n))
+        return bail(set_error(MSOSM_ERR_UNSUPPORTED, "fused forms support FFT sizes 4..16384 (got %lu); use "
+                                                     "MSOSM_FORM_PLAIN_F32", N));
+    int ndev = 0;
+    hipError_t herr = hipGetDeviceCount(&ndev);
+    if (herr != hipSuccess || ndev < 1)
+        return bail(set_error(MSOSM_ERR_HIP, "no HIP device available (%s)", hipGetErrorString(herr)));
+    if (stream) {
+        e->stream = (hipStream_t)stream;
+    } else {
+        herr = hipStreamCreate(&e->stream);
+        if (herr != hipSuccess)
+            return bail(set_error(MSOSM_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(herr)));
+        e->own_stream = true;
+    }
+#define ALLOC(ptr, n)                                                                                            \
+    do {                                                                                                         \
+        hipError_t a__ = dev_alloc(&(ptr), (n));                                                                 \
+        if (a__ != hipSuccess)                                                                                   \
+            return bail(set_error(MSOSM_ERR_ALLOC, "hipMalloc %s (%zu bytes): %s", #ptr,                         \
+                                  (size_t)(n) * sizeof(*(ptr)), hipGetErrorString(a__)));                       \
+    } while (0)
+    const size_t out_elems = (size_t)ndm * count * M + 1;
+    ALLOC(e->in_stage, (size_t)count * M);
+    ALLOC(e->H32, (size_t)ndm * N);
+    ALLOC(e->out, out_elems);
+    ALLOC(e->out_u16, out_elems);
+    if (e->form == MSOSM_FORM_PLAIN_F32) {
+        ALLOC(e->delay32, (size_t)ndm * N);
+        ALLOC(e->inbuf, (size_t)(count + 1) * M);
+        ALLOC(e->fftblk, (size_t)count * N);
+        ALLOC(e->ring, (size_t)ndm * p.R * N);
+        ALLOC(e->ifftblk, (size_t)ndm * count * N);
+        size_t sc = std::max(msosm::fft_c2c_scratch_elems(e->log2n, count),
+                             msosm::fft_c2c_scratch_elems(e->log2n, (long long)ndm * count));
+        if (sc)
+            ALLOC(e->fft_scratch, sc);
+        HIP_TRY(hipMemsetAsync(e->inbuf, 0, sizeof(float2) * (count + 1) * M, e->stream));
+        HIP_TRY(hipMemsetAsync(e->ring, 0, sizeof(float2) * (size_t)ndm * p.R * N, e->stream));
+        HIP_TRY(hipMemcpyAsync(e->delay32, p.delay.data(), sizeof(int32_t) * ndm * N, hipMemcpyHostToDevice,
+                               e->stream));
+    } else {
+        const bool f16 = e->form == MSOSM_FORM_FUSED_F16;
+        e->rx = p.K_max - 1 + count;
+        ALLOC(e->delay16, (size_t)ndm * N);
+        ALLOC(e->tail, M);
+        if (f16) {
+            ALLOC(e->H16, (size_t)ndm * N);
+            __half2 *r = nullptr;
+            ALLOC(r, (size_t)e->rx * N);
+            e->xring = r;
+            HIP_TRY(hipMemsetAsync(e->xring, 0, sizeof(__half2) * (size_t)e->rx * N, e->stream));
+        } else {
+            float2 *r = nullptr;
+            ALLOC(r, (size_t)e->rx * N);
+            e->xring = r;
+            HIP_TRY(hipMemsetAsync(e->xring, 0, sizeof(float2) * (size_t)e->rx * N, e->stream));
+        }
+        HIP_TRY(hipMemsetAsync(e->tail, 0, sizeof(uint32_t) * M, e->stream));
+        int32_t *tmp = nullptr;
+        ALLOC(tmp, (size_t)ndm * N);
+        herr = hipMemcpyAsync(tmp, p.delay.data(), sizeof(int32_t) * ndm * N, hipMemcpyHostToDevice, e->stream);
+        if (herr == hipSuccess)
+            herr = msosm::launch_i32_to_u16(e->delay16, tmp, (size_t)ndm * N, e->stream);
+        hipStreamSynchronize(e->stream);
+        hipFree(tmp);
+        if (herr != hipSuccess)
+            return bail(set_error(MSOSM_ERR_HIP, "delay table upload: %s", hipGetErrorString(herr)));
+    }
+#undef ALLOC
+    HIP_TRY(hipMemsetAsync(e->out, 0, sizeof(float2) * out_elems, e->stream));
+    HIP_TRY(hipMemsetAsync(e->out_u16, 0, sizeof(uint32_t) * out_elems, e->stream));
+    int rc = build_filter_tables(e);
+    if (rc != MSOSM_OK)
+        return bail(rc);
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    *out = e;
+    return MSOSM_OK;
+}
+
+int filter_plain(msosm_engine *e, const uint32_t *cur)
+{
+    const msosm::Plan &p = e->plan;
+    const unsigned long M = p.M, N = p.N;
+    const int ndm = p.num_dms, count = p.count;
+    hipStream_t s = e->stream;
+    {
+        StageScope st(e, MSOSM_STAGE_COPY); // overlap-save history (msosm_gpu_batch.cpp:80)
+        HIP_TRY(hipMemcpyAsync(e->inbuf, e->inbuf + (size_t)count * M, sizeof(float2) * M, hipMemcpyDeviceToDevice, s));
+    }
+    {
+        StageScope st(e, MSOSM_STAGE_UNPACK);
+        HIP_TRY(msosm::launch_unpack(e->inbuf + M, cur, (size_t)count * M, s));
+    }
+    {
+        StageScope st(e, MSOSM_STAGE_FFT_FWD);
+        HIP_TRY(msosm::launch_fft_c2c(e->inbuf, e->fftblk, e->log2n, count, M, N, false, 1.0f, e->fft_scratch, s));
+    }
+    {
+        StageScope st(e, MSOSM_STAGE_MULTIPLY);
+        HIP_TRY(msosm::launch_multiply_delay(e->fftblk, e->H32, (int)N, e->ring, e->delay32, p.R, e->current_index,
+                                             count, ndm, s));
+    }
+    {
+        StageScope st(e, MSOSM_STAGE_FFT_INV);
+        const float2 *grp = e->ring + (size_t)e->current_index * N * ndm;
+        HIP_TRY(msosm::launch_fft_c2c(grp, e->ifftblk, e->log2n, (long long)ndm * count, N, N, true, 1.0f,
+                                      e->fft_scratch, s));
+    }
+    e->current_index += count; // msosm_gpu_dm_concurrent.cpp:142-146
+    if (e->current_index >= p.R)
+        e->current_index -= p.R;
+    {
+        StageScope st(e, MSOSM_STAGE_DISCARD);
+        HIP_TRY(msosm::launch_discard(e->ifftblk, e->out, (int)M, count, ndm, s));
+    }
+    return MSOSM_OK;
+}
+
+int filter_fused(msosm_engine *e, const uint32_t *cur)
+{
+    const msosm::Plan &p = e->plan;
+    const unsigned long M = p.M;
+    const int ndm = p.num_dms, count = p.count;
+    const bool f16 = e->form == MSOSM_FORM_FUSED_F16;
+    hipStream_t s = e->stream;
+    {
+        StageScope st(e, MSOSM_STAGE_FFT_FWD);
+        HIP_TRY(msosm::launch_fwd_u16(cur, e->tail, e->xring, f16, e->log2n, count, e->g0, e->rx, s));
+    }
+    {
+        StageScope st(e, MSOSM_STAGE_COPY);
+        HIP_TRY(hipMemcpyAsync(e->tail, cur + (size_t)(count - 1) * M, sizeof(uint32_t) * M, hipMemcpyDeviceToDevice,
+                               s));
+    }
+    {
+        StageScope st(e, MSOSM_STAGE_FFT_INV);
+        HIP_TRY(msosm::launch_inv_fused(e->xring, f16 ? (const void *)e->H16 : (const void *)e->H32, e->delay16,
+                                        e->out, f16, e->log2n, count, ndm, e->g0, e->rx, s));
+    }
+    e->g0 += count;
+    return MSOSM_OK;
+}
+
+} // namespace
+
+// ===========================================================================
+// C ABI
+
+extern "C" {
+
+const char *msosm_last_error(void) { return g_last_error.c_str(); }
+const char *msosm_version(void) { return "msosm-mi355x 0.1 (gfx950)"; }
+
+int msosm_plan_create(const msosm_plan_desc *desc, msosm_plan **out)
+{
+    if (!desc || !out)
+        return set_error(MSOSM_ERR_ARG, "plan_create: NULL argument");
+    *out = nullptr;
+    msosm_plan *p = new msosm_plan();
+    std::string msg;
+    int rc = msosm::build_plan(*desc, p->p, msg);
+    if (rc != MSOSM_OK) {
+        delete p;
+        return set_error(rc, "%s", msg.c_str());
+    }
+    *out = p;
+    return MSOSM_OK;
+}
+
+void msosm_plan_destroy(msosm_plan *plan) { delete plan; }
+
+int msosm_plan_get_info(const msosm_plan *plan, msosm_plan_info *info)
+{
+    if (!plan || !info)
+        return set_error(MSOSM_ERR_ARG, "plan_get_info: NULL argument");
+    const msosm::Plan &p = plan->p;
+    info->M = p.M;
+    info->M_min = p.M_min;
+    info->fftpoint = p.N;
+    info->count = p.count;
+    info->num_dms = p.num_dms;
+    info->K_max = p.K_max;
+    info->R = p.R;
+    info->clamped = p.clamped;
+    return MSOSM_OK;
+}
+
+int msosm_plan_dm_info(const msosm_plan *plan, int i, unsigned long *nd, unsigned long *m_min, int *delaycount,
+                       int *delay_block_size)
+{
+    if (!plan || i < 0 || i >= plan->p.num_dms)
+        return set_error(MSOSM_ERR_ARG, "plan_dm_info: bad plan or DM index %d", i);
+    const msosm::Plan &p = plan->p;
+    if (nd)
+        *nd = p.dms[i].nd;
+    if (m_min)
+        *m_min = p.m_min[i];
+    if (delaycount)
+        *delaycount = p.K[i];
+    if (delay_block_size)
+        *delay_block_size = p.delay_block_size[i];
+    return MSOSM_OK;
+}
+
+int msosm_plan_delay_points(const msosm_plan *plan, int i, int32_t *out)
+{
+    if (!plan || !out || i < 0 || i >= plan->p.num_dms)
+        return set_error(MSOSM_ERR_ARG, "plan_delay_points: bad argument");
+    const msosm::Plan &p = plan->p;
+    std::memcpy(out, p.delay.data() + (size_t)i * p.N, sizeof(int32_t) * p.N);
+    return MSOSM_OK;
+}
+
+int msosm_engine_create(const msosm_plan *plan, void *hip_stream, msosm_engine **out)
+{
+    if (!plan || !out)
+        return set_error(MSOSM_ERR_ARG, "engine_create: NULL argument");
+    *out = nullptr;
+    return create_engine(plan->p, hip_stream, out);
+}
+
+int msosm_engine_destroy(msosm_engine *eng)
+{
+    free_engine(eng);
+    return MSOSM_OK;
+}
+
+int msosm_engine_filter_u16(msosm_engine *e, const void *input, int where)
+{
+    if (!e || !input)
+        return set_error(MSOSM_ERR_ARG, "filter_u16: NULL argument");
+    const msosm::Plan &p = e->plan;
+    const uint32_t *cur = (const uint32_t *)input;
+    if (where == 0) {
+        StageScope st(e, MSOSM_STAGE_COPY);
+        HIP_TRY(hipMemcpyAsync(e->in_stage, input, sizeof(uint32_t) * (size_t)p.count * p.M, hipMemcpyHostToDevice,
+                               e->stream));
+        cur = e->in_stage;
+    } else if (where != 1) {
+        return set_error(MSOSM_ERR_ARG, "filter_u16: where must be 0 (host) or 1 (device)");
+    }
+    int rc = (e->form == MSOSM_FORM_PLAIN_F32) ? filter_plain(e, cur) : filter_fused(e, cur);
+    if (rc == MSOSM_OK)
+        e->have_output = true;
+    return rc;
+}
+
+int msosm_engine_output_c64(msosm_engine *e, void *dst, int where)
+{
+    if (!e || !dst)
+        return set_error(MSOSM_ERR_ARG, "output_c64: NULL argument");
+    const msosm::Plan &p = e->plan;
+    size_t bytes = sizeof(float2) * (size_t)p.num_dms * p.count * p.M;
+    HIP_TRY(hipMemcpyAsync(dst, e->out, bytes, where == 1 ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+                           e->stream));
+    if (where != 1)
+        HIP_TRY(hipStreamSynchronize(e->stream));
+    return MSOSM_OK;
+}
+
+int msosm_engine_output_u16(msosm_engine *e, void *dst, int where)
+{
+    if (!e || !dst)
+        return set_error(MSOSM_ERR_ARG, "output_u16: NULL argument");
+    const msosm::Plan &p = e->plan;
+    size_t n = (size_t)p.num_dms * p.count * p.M;
+    HIP_TRY(msosm::launch_pack(e->out_u16, e->out, n, e->stream));
+    HIP_TRY(hipMemcpyAsync(dst, e->out_u16, sizeof(uint32_t) * n,
+                           where == 1 ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, e->stream));
+    if (where != 1)
+        HIP_TRY(hipStreamSynchronize(e->stream));
+    return MSOSM_OK;
+}
+
+int msosm_engine_output_ptr(msosm_engine *e, void **dev_ptr)
+{
+    if (!e || !dev_ptr)
+        return set_error(MSOSM_ERR_ARG, "output_ptr: NULL argument");
+    *dev_ptr = e->out;
+    return MSOSM_OK;
+}
+
+int msosm_engine_dedisp_params(msosm_engine *e, int i, float *host_out)
+{
+    if (!e || !host_out || i < 0 || i >= e->plan.num_dms)
+        return set_error(MSOSM_ERR_ARG, "dedisp_params: bad argument");
+    HIP_TRY(hipMemcpyAsync(host_out, e->H32 + (size_t)i * e->plan.N, sizeof(float2) * e->plan.N,
+                           hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    return MSOSM_OK;
+}
+
+int msosm_engine_synchronize(msosm_engine *e)
+{
+    if (!e)
+        return set_error(MSOSM_ERR_ARG, "synchronize: NULL engine");
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    return MSOSM_OK;
+}
+
+int msosm_engine_reset(msosm_engine *e)
+{
+    if (!e)
+        return set_error(MSOSM_ERR_ARG, "reset: NULL engine");
+    const msosm::Plan &p = e->plan;
+    if (e->form == MSOSM_FORM_PLAIN_F32) {
+        HIP_TRY(hipMemsetAsync(e->inbuf, 0, sizeof(float2) * (p.count + 1) * p.M, e->stream));
+        HIP_TRY(hipMemsetAsync(e->ring, 0, sizeof(float2) * (size_t)p.num_dms * p.R * p.N, e->stream));
+        e->current_index = 0;
+    } else {
+        size_t esz = e->form == MSOSM_FORM_FUSED_F16 ? sizeof(__half2) : sizeof(float2);
+        HIP_TRY(hipMemsetAsync(e->xring, 0, esz * (size_t)e->rx * p.N, e->stream));
+        HIP_TRY(hipMemsetAsync(e->tail, 0, sizeof(uint32_t) * p.M, e->stream));
+        e->g0 = 0;
+    }
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    return MSOSM_OK;
+}
+
+int msosm_engine_stream(msosm_engine *e, void **s)
+{
+    if (!e || !s)
+        return set_error(MSOSM_ERR_ARG, "stream: NULL argument");
+    *s = (void *)e->stream;
+    return MSOSM_OK;
+}
+
+int msosm_engine_set_timing(msosm_engine *e, int on)
+{
+    if (!e)
+        return set_error(MSOSM_ERR_ARG, "set_timing: NULL engine");
+    harvest(e);
+    e->timing = on != 0;
+    for (int i = 0; i < MSOSM_NUM_STAGES; i++) {
+        e->stage_ms[i] = 0;
+        e->stage_calls[i] = 0;
+    }
+    return MSOSM_OK;
+}
+
+int msosm_engine_get_timing(msosm_engine *e, double *times_ms, long long *calls, int n)
+{
+    if (!e)
+        return set_error(MSOSM_ERR_ARG, "get_timing: NULL engine");
+    harvest(e);
+    for (int i = 0; i < n && i < MSOSM_NUM_STAGES; i++) {
+        if (times_ms)
+            times_ms[i] = e->stage_ms[i];
+        if (calls)
+            calls[i] = e->stage_calls[i];
+    }
+    return MSOSM_OK;
+}
+
+// ---- element kernels -------------------------------------------------------
+int msosm_k_uint16_to_complex(float *dst, const void *src, size_t n, void *stream)
+{
+    HIP_TRY(msosm::launch_unpack((float2 *)dst, (const uint32_t *)src, n, (hipStream_t)stream));
+    return MSOSM_OK;
+}
+
+int msosm_k_complex_to_uint16(void *dst, const float *src, size_t n, void *stream)
+{
+    HIP_TRY(msosm::launch_pack((uint32_t *)dst, (const float2 *)src, n, (hipStream_t)stream));
+    return MSOSM_OK;
+}
+
+int msosm_k_intensity(const float *a, const float *b, float *out, size_t n, void *stream)
+{
+    HIP_TRY(msosm::launch_intensity((const float2 *)a, (const float2 *)b, out, n, (hipStream_t)stream));
+    return MSOSM_OK;
+}
+
+int msosm_k_multiply_batch(const float *a, const float *b, int N, float *out, int count, int ndm, void *stream)
+{
+    if (N < 1 || count < 1 || ndm < 1)
+        return set_error(MSOSM_ERR_ARG, "multiply_batch: bad sizes");
+    HIP_TRY(msosm::launch_multiply((const float2 *)a, (const float2 *)b, N, (float2 *)out, count, ndm,
+                                   (hipStream_t)stream));
+    return MSOSM_OK;
+}
+
+int msosm_k_multiply_delay_batch(const float *a, const float *b, int N, float *ring, const int32_t *delay, int dbs,
+                                 int index, int count, int ndm, void *stream)
+{
+    if (N < 1 || count < 1 || ndm < 1 || dbs < 1 || dbs % count != 0 || index < 0)
+        return set_error(MSOSM_ERR_ARG, "multiply_delay_batch: bad sizes (delay_block_size must be a positive "
+                                        "multiple of count)");
+    HIP_TRY(msosm::launch_multiply_delay((const float2 *)a, (const float2 *)b, N, (float2 *)ring, delay, dbs, index,
+                                         count, ndm, (hipStream_t)stream));
+    return MSOSM_OK;
+}
+
+int msosm_k_discard_batch(const float *a, float *b, int M, int count, int ndm, void *stream)
+{
+    if (M < 1 || count < 1 || ndm < 1)
+        return set_error(MSOSM_ERR_ARG, "discard_batch: bad sizes");
+    HIP_TRY(msosm::launch_discard((const float2 *)a, (float2 *)b, M, count, ndm, (hipStream_t)stream));
+    return MSOSM_OK;
+}
+
+int msosm_k_fft_c2c(const float *in, float *out, size_t n, size_t batch, size_t idist, size_t odist, int direction,
+                    void *stream)
+{
+    if (n < 2 || (n & (n - 1)) || n > (1ul << 28) || batch < 1 || (direction != -1 && direction != 1))
+        return set_error(MSOSM_ERR_ARG, "fft_c2c: n must be a power of two in [2, 2^28], direction -1 or +1");
+    HIP_TRY(msosm::launch_fft_c2c((const float2 *)in, (float2 *)out, ilog2_exact(n), (long long)batch,
+                                  (long long)idist, (long long)odist, direction == 1, 1.0f, nullptr,
+                                  (hipStream_t)stream));
+    return MSOSM_OK;
+}
+
+} // extern "C"

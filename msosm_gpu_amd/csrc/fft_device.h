@@ -1,0 +1,287 @@
+// fft_device.h -- hand-written CDNA4 FFT building blocks (device side).
+//
+// One transform of N = 2^LOG2N complex points is spread over NT = N/E threads
+// of a workgroup; thread t holds the E points t + m*NT (m = 0..E-1) in
+// registers ("cyclic" layout) on entry and the E outputs with the same
+// indices on exit.  Inside, the transform is a Stockham autosort FFT (the
+// radix-R formulation of Govindaraju et al., "High performance discrete
+// Fourier transforms on graphics processors", SC08): every pass does radix-R
+// butterflies in registers and exchanges through LDS; the last pass may use a
+// smaller radix.  Callers fuse their own loads (unpack, gather x filter) and
+// stores (discard, scale, FP16 packing) around fft_block().
+//
+// Twiddles W_N^e = exp(-2 pi i e/N) come from two small LDS tables
+// (W = hi[e >> LS] * lo[e & (2^LS-1)]) filled once per workgroup in FP64.
+// LDS addresses are padded by one element every E elements so the strided
+// Stockham writes do not all land in the same banks.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+
+namespace msosm {
+
+__host__ __device__ constexpr int ilog2c(int v) { return v <= 1 ? 0 : 1 + ilog2c(v >> 1); }
+
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b)
+{
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cmulconj(float2 a, float2 b) // a * conj(b)
+{
+    return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+
+// cos(2 pi k / 16) for k = 0..15, folded at compile time.
+__host__ __device__ constexpr float cos16(int k)
+{
+    k &= 15;
+    return k == 0 ? 1.0f
+         : k == 1 ? 0.92387953251128674f
+         : k == 2 ? 0.70710678118654752f
+         : k == 3 ? 0.38268343236508977f
+         : k == 4 ? 0.0f
+         : k == 5 ? -0.38268343236508977f
+         : k == 6 ? -0.70710678118654752f
+         : k == 7 ? -0.92387953251128674f
+         : k == 8 ? -1.0f
+         : k == 9 ? -0.92387953251128674f
+         : k == 10 ? -0.70710678118654752f
+         : k == 11 ? -0.38268343236508977f
+         : k == 12 ? 0.0f
+         : k == 13 ? 0.38268343236508977f
+         : k == 14 ? 0.70710678118654752f
+                   : 0.92387953251128674f;
+}
+__host__ __device__ constexpr float sin16(int k) { return cos16(k + 12); }
+
+// Multiply by W_R^e (forward: exp(-2 pi i e/R), inverse: conjugate), R <= 16,
+// with the trivial angles resolved at compile time.
+template <int R, bool INV, int E_>
+__device__ __forceinline__ float2 twiddle_const(float2 a)
+{
+    constexpr int k = ((E_ % R) * (16 / R)) & 15; // angle index in 16ths
+    if constexpr (k == 0) {
+        return a;
+    } else if constexpr (k == 4) { // -i (fwd) / +i (inv)
+        return INV ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+    } else if constexpr (k == 8) {
+        return make_float2(-a.x, -a.y);
+    } else if constexpr (k == 12) { // +i (fwd) / -i (inv)
+        return INV ? make_float2(a.y, -a.x) : make_float2(-a.y, a.x);
+    } else {
+        constexpr float c = cos16(k);
+        constexpr float s = INV ? sin16(k) : -sin16(k);
+        return make_float2(a.x * c - a.y * s, a.x * s + a.y * c);
+    }
+}
+
+// In-register DFT of size R (natural order in and out).
+template <int R, bool INV>
+struct Dft;
+
+template <bool INV>
+struct Dft<1, INV> {
+    __device__ __forceinline__ static void run(float2 *) {}
+};
+
+template <bool INV>
+struct Dft<2, INV> {
+    __device__ __forceinline__ static void run(float2 *v)
+    {
+        float2 a = v[0], b = v[1];
+        v[0] = cadd(a, b);
+        v[1] = csub(a, b);
+    }
+};
+
+template <bool INV>
+struct Dft<4, INV> {
+    __device__ __forceinline__ static void run(float2 *v)
+    {
+        float2 s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
+        float2 s13 = cadd(v[1], v[3]), d13 = csub(v[1], v[3]);
+        // forward: X1 = d02 - i d13, X3 = d02 + i d13
+        float2 jd13 = INV ? make_float2(-d13.y, d13.x) : make_float2(d13.y, -d13.x);
+        v[0] = cadd(s02, s13);
+        v[2] = csub(s02, s13);
+        v[1] = cadd(d02, jd13);
+        v[3] = csub(d02, jd13);
+    }
+};
+
+// R = A*B with A = 4: n = B*n1 + n2, k = k1 + A*k2.
+template <int R, bool INV>
+struct Dft {
+    static constexpr int A = 4;
+    static constexpr int B = R / A;
+    template <int N2>
+    __device__ __forceinline__ static void stage1(float2 *v)
+    {
+        float2 u[A];
+#pragma unroll
+        for (int n1 = 0; n1 < A; n1++)
+            u[n1] = v[B * n1 + N2];
+        Dft<A, INV>::run(u);
+        v[B * 0 + N2] = u[0];
+        v[B * 1 + N2] = twiddle_const<R, INV, N2 * 1>(u[1]);
+        v[B * 2 + N2] = twiddle_const<R, INV, N2 * 2>(u[2]);
+        v[B * 3 + N2] = twiddle_const<R, INV, N2 * 3>(u[3]);
+    }
+    template <int N2>
+    __device__ __forceinline__ static void stage1_all(float2 *v)
+    {
+        if constexpr (N2 < B) {
+            stage1<N2>(v);
+            stage1_all<N2 + 1>(v);
+        }
+    }
+    __device__ __forceinline__ static void run(float2 *v)
+    {
+        stage1_all<0>(v);
+        // now v[B*k1 + n2] holds twiddled Y[n2][k1]; DFT_B over n2 for each k1
+        float2 out[R];
+#pragma unroll
+        for (int k1 = 0; k1 < A; k1++) {
+            float2 w[B];
+#pragma unroll
+            for (int n2 = 0; n2 < B; n2++)
+                w[n2] = v[B * k1 + n2];
+            Dft<B, INV>::run(w);
+#pragma unroll
+            for (int k2 = 0; k2 < B; k2++)
+                out[k1 + A * k2] = w[k2];
+        }
+#pragma unroll
+        for (int i = 0; i < R; i++)
+            v[i] = out[i];
+    }
+};
+
+// Compile-time shape of one transform.
+template <int LOG2N, int E>
+struct FftShape {
+    static constexpr int N = 1 << LOG2N;
+    static constexpr int LOGE = ilog2c(E);
+    static constexpr int NT = N / E;                       // threads per transform
+    static constexpr int NPASS = (LOG2N + LOGE - 1) / LOGE; // Stockham passes
+    static constexpr int LS = (LOG2N + 1) / 2;              // low twiddle bits
+    static constexpr int TWLO = 1 << LS;
+    static constexpr int TWHI = 1 << (LOG2N - LS);
+    static constexpr int NPAD = N + (N >> LOGE);            // padded LDS length
+};
+
+template <int LOGE>
+__device__ __forceinline__ int lidx(int i) { return i + (i >> LOGE); }
+
+// Fill the two twiddle tables (forward sign) for a transform of 2^LOG2N.
+template <int LOG2N, int E>
+__device__ __forceinline__ void fill_twiddles(float2 *twlo, float2 *twhi, int tid, int nthreads)
+{
+    using S = FftShape<LOG2N, E>;
+    for (int i = tid; i < S::TWLO + S::TWHI; i += nthreads) {
+        double frac;
+        float2 *dst;
+        if (i < S::TWLO) {
+            frac = (double)i / (double)S::N;
+            dst = twlo + i;
+        } else {
+            int k = i - S::TWLO;
+            frac = (double)k * (double)S::TWLO / (double)S::N;
+            dst = twhi + k;
+        }
+        double s, c;
+        sincospi(2.0 * frac, &s, &c);
+        *dst = make_float2((float)c, (float)(-s));
+    }
+}
+
+template <int LOG2N, int E, bool INV>
+__device__ __forceinline__ float2 twiddle_lookup(const float2 *twlo, const float2 *twhi, int e)
+{
+    using S = FftShape<LOG2N, E>;
+    float2 w = cmul(twhi[e >> S::LS], twlo[e & (S::TWLO - 1)]);
+    return INV ? make_float2(w.x, -w.y) : w;
+}
+
+template <int LOG2N, int E, bool INV, int P>
+__device__ __forceinline__ void fft_pass(float2 (&v)[E], float2 *lds, const float2 *twlo, const float2 *twhi, int t)
+{
+    using S = FftShape<LOG2N, E>;
+    constexpr int N = S::N, NT = S::NT, LOGE = S::LOGE, NPASS = S::NPASS;
+    constexpr int LOGR = (P < NPASS - 1) ? LOGE : (LOG2N - (NPASS - 1) * LOGE);
+    constexpr int R = 1 << LOGR;
+    constexpr int LOGNS = P * LOGE;
+    constexpr int NS = 1 << LOGNS;
+    constexpr int Q = E / R;
+    if constexpr (P > 0) {
+#pragma unroll
+        for (int q = 0; q < Q; q++)
+#pragma unroll
+            for (int r = 0; r < R; r++)
+                v[q + r * Q] = lds[lidx<LOGE>(t + q * NT + r * (N / R))];
+        if constexpr (P < NPASS - 1)
+            __syncthreads();
+    }
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        float2 u[R];
+#pragma unroll
+        for (int r = 0; r < R; r++)
+            u[r] = v[q + r * Q];
+        const int j = t + q * NT;
+        const int k = j & (NS - 1);
+        if constexpr (P > 0) {
+#pragma unroll
+            for (int r = 1; r < R; r++) {
+                int e = (r * k) << (LOG2N - LOGNS - LOGR);
+                u[r] = cmul(u[r], twiddle_lookup<LOG2N, E, INV>(twlo, twhi, e));
+            }
+        }
+        Dft<R, INV>::run(u);
+        if constexpr (P == NPASS - 1) {
+#pragma unroll
+            for (int r = 0; r < R; r++)
+                v[q + r * Q] = u[r];
+        } else {
+            const int base = ((j >> LOGNS) << (LOGNS + LOGR)) + k;
+#pragma unroll
+            for (int r = 0; r < R; r++)
+                lds[lidx<LOGE>(base + r * NS)] = u[r];
+        }
+    }
+    if constexpr (P < NPASS - 1) {
+        __syncthreads();
+        fft_pass<LOG2N, E, INV, P + 1>(v, lds, twlo, twhi, t);
+    }
+}
+
+// Full transform.  Every thread of the workgroup must call it (it contains
+// barriers).  lds must hold FftShape::NPAD float2 for this transform.  On
+// entry v[m] = x[t + m*NT]; on exit v[m] = X[t + m*NT] (unnormalised).
+template <int LOG2N, int E, bool INV>
+__device__ __forceinline__ void fft_block(float2 (&v)[E], float2 *lds, const float2 *twlo, const float2 *twhi, int t)
+{
+    using S = FftShape<LOG2N, E>;
+    if constexpr (S::NPASS == 1) {
+        Dft<S::N, INV>::run(v);
+    } else {
+        __syncthreads(); // LDS may still be read by a previous transform
+        fft_pass<LOG2N, E, INV, 0>(v, lds, twlo, twhi, t);
+    }
+}
+
+// Elements per thread for a transform size.
+__host__ __device__ constexpr int fft_e(int log2n) { return log2n >= 4 ? 16 : (1 << log2n); }
+// Transforms per workgroup (fill ~256 threads for small sizes).
+__host__ __device__ constexpr int fft_tpb(int log2n)
+{
+    return ((1 << log2n) / fft_e(log2n)) >= 256 ? 1 : 256 / ((1 << log2n) / fft_e(log2n));
+}
+
+// Threads of a workgroup running fft_tpb() transforms.
+__host__ __device__ constexpr int fft_threads(int log2n) { return ((1 << log2n) / fft_e(log2n)) * fft_tpb(log2n); }
+
+} // namespace msosm

@@ -1,0 +1,51 @@
+// kernels.h -- host launchers of the HIP kernels (kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+#include <cstddef>
+#include <cstdint>
+
+namespace msosm {
+
+constexpr int kMaxLog2SinglePass = 14; // one workgroup holds 2^14 complex in LDS
+
+// Batched C2C FFT, any power of two 2^1..2^28 (one pass up to 2^14, two
+// passes above).  scale multiplies the output.  scratch: for two-pass sizes a
+// buffer of batch*n float2 (NULL: the launcher allocates).
+hipError_t launch_fft_c2c(const float2 *in, float2 *out, int log2n, long long batch, long long idist,
+                          long long odist, bool inverse, float scale, float2 *scratch, hipStream_t s);
+size_t fft_c2c_scratch_elems(int log2n, long long batch);
+
+// Fused forward FFT of count overlapped windows read straight from packed
+// uint16 input (cur) and the previous call's last M samples (tail), written
+// to slots (g0 + w) mod rx of a spectra ring (FP32, or FP16 scaled by 1/N).
+hipError_t launch_fwd_u16(const uint32_t *cur, const uint32_t *tail, void *ring, bool f16, int log2n, int count,
+                          long long g0, int rx, hipStream_t s);
+// Fused inverse: per (DM, block) gather X_{g-d(n)}[n] * H[n] from the ring,
+// inverse FFT, keep the last M samples, scale; out layout [dm][count][M].
+hipError_t launch_inv_fused(const void *ring, const void *H, const uint16_t *delay, float2 *out, bool f16,
+                            int log2n, int count, int ndm, long long g0, int rx, hipStream_t s);
+bool fused_supported(int log2n);
+
+// Element kernels (reference gpu_kernel.cu semantics).
+hipError_t launch_unpack(float2 *dst, const uint32_t *src, size_t n, hipStream_t s);
+hipError_t launch_pack(uint32_t *dst, const float2 *src, size_t n, hipStream_t s);
+hipError_t launch_intensity(const float2 *a, const float2 *b, float *out, size_t n, hipStream_t s);
+hipError_t launch_multiply(const float2 *a, const float2 *b, int N, float2 *out, int count, int ndm, hipStream_t s);
+hipError_t launch_multiply_delay(const float2 *a, const float2 *b, int N, float2 *ring, const int32_t *delay,
+                                 int dbs, int index, int count, int ndm, hipStream_t s);
+hipError_t launch_discard(const float2 *a, float2 *b, int M, int count, int ndm, hipStream_t s);
+
+// Filter-table generation: chirp samples (FP64 phase) in IDFT input order.
+struct ChirpParams {
+    float fs, step, w0, kdm, dm, f0;
+};
+hipError_t launch_chirp(float2 *c, unsigned long M, int mode, const ChirpParams *params_dev, int ndm,
+                        hipStream_t s);
+// dst[d][0..2M) = (src[d][0..M), 0...)
+hipError_t launch_zero_pad(float2 *dst, const float2 *src, unsigned long M, int ndm, hipStream_t s);
+// dst = half2(src * scale)
+hipError_t launch_to_half(__half2 *dst, const float2 *src, size_t n, float scale, hipStream_t s);
+hipError_t launch_i32_to_u16(uint16_t *dst, const int32_t *src, size_t n, hipStream_t s);
+
+} // namespace msosm

@@ -1,0 +1,733 @@
+// kernels.hip -- hand-written CDNA4 (gfx950) kernels of the MS-OSM engine.
+//
+//  * element kernels with the semantics of the reference's gpu/gpu_kernel.cu
+//    (unpack, pack, intensity, multiply, multiply-delay scatter, discard);
+//  * batched C2C FFTs: one LDS pass up to 2^14 points, a two-pass four-step
+//    (columns then rows, inter-pass twiddle in FP64) above;
+//  * the fused forms: unpack + forward FFT of overlapped windows straight into
+//    a shared spectra ring, and gather X_{g-d(n)}[n]*H[n] + inverse FFT +
+//    discard + scale in one kernel;
+//  * the FP64 chirp generator for the filter tables.
+#include "kernels.h"
+
+#include <hip/hip_fp16.h>
+
+#include "fft_device.h"
+
+namespace msosm {
+
+static constexpr int kBlock = 256;
+
+static inline int grid_for(size_t n, int per_thread = 1)
+{
+    size_t g = (n + (size_t)kBlock * per_thread - 1) / ((size_t)kBlock * per_thread);
+    if (g < 1)
+        g = 1;
+    if (g > 1048576)
+        g = 1048576; // grid-stride loops cover the rest
+    return (int)g;
+}
+
+// ---------------------------------------------------------------------------
+// Element kernels
+
+// gpu_kernel.cu:276-299 -- offset binary to float, code 0 ("no data") -> 0.
+__device__ __forceinline__ float2 unpack_u16pair(uint32_t u)
+{
+    uint32_t x = u & 0xFFFFu, y = u >> 16;
+    float fx = (float)x - (x != 0 ? 32768.0f : 0.0f);
+    float fy = (float)y - (y != 0 ? 32768.0f : 0.0f);
+    return make_float2(fx, fy);
+}
+
+// gpu_kernel.cu:317-329 -- truncating conversion after adding 32768 to
+// non-zero values; saturates to [0, 65535], NaN -> 0.
+__device__ __forceinline__ uint32_t pack_one(float v)
+{
+#pragma clang fp contract(off)
+    float t = v + (v != 0.0f ? 32768.0f : 0.0f);
+    uint32_t u;
+    if (!(t > 0.0f))
+        u = 0;
+    else if (t >= 65535.0f)
+        u = 65535;
+    else
+        u = (uint32_t)t;
+    return u;
+}
+
+__global__ void k_unpack(float2 *__restrict__ dst, const uint32_t *__restrict__ src, size_t n)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = unpack_u16pair(src[i]);
+}
+
+__global__ void k_pack(uint32_t *__restrict__ dst, const float2 *__restrict__ src, size_t n)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        float2 v = src[i];
+        dst[i] = pack_one(v.x) | (pack_one(v.y) << 16);
+    }
+}
+
+// gpu_kernel.cu:347-353 -- sqrt(|a|^2 + |b|^2), summed left to right, no
+// fused multiply-add (so a CPU restatement reproduces it bit for bit).
+__global__ void k_intensity(const float2 *__restrict__ a, const float2 *__restrict__ b, float *__restrict__ out,
+                            size_t n)
+{
+#pragma clang fp contract(off)
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        float2 p = a[i];
+        float s = p.x * p.x;
+        s = s + p.y * p.y;
+        if (b) {
+            float2 q = b[i];
+            s = s + q.x * q.x;
+            s = s + q.y * q.y;
+        }
+        out[i] = __fsqrt_rn(s);
+    }
+}
+
+// gpu_kernel.cu:56-66 -- out[dm][c][n] = (a[c][n]/N) * b[dm][n]
+__global__ void k_multiply(const float2 *__restrict__ a, const float2 *__restrict__ b, float2 *__restrict__ out,
+                           int N, int count, int ndm)
+{
+    const size_t per = (size_t)N * count, total = per * ndm;
+    const float invN = 1.0f / (float)N;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+        size_t dm = i / per, r = i % per;
+        float2 x = a[r];
+        float2 h = b[dm * N + (r % N)];
+        x = make_float2(x.x * invN, x.y * invN);
+        out[i] = cmul(x, h);
+    }
+}
+
+// gpu_kernel.cu:160-197 -- scatter the product of block c into ring slot
+// j = (index + c + d[dm][n]) % dbs, layout [j/count][dm][j%count][N].
+__global__ void k_multiply_delay(const float2 *__restrict__ a, const float2 *__restrict__ b, float2 *__restrict__ ring,
+                                 const int32_t *__restrict__ delay, int dbs, int index, int N, int count, int ndm)
+{
+    const float invN = 1.0f / (float)N;
+    const int dm = blockIdx.x % ndm;
+    const int c = blockIdx.x / ndm;
+    const float2 *arow = a + (size_t)c * N;
+    const float2 *brow = b + (size_t)dm * N;
+    const int32_t *drow = delay + (size_t)dm * N;
+    for (int n = blockIdx.y * blockDim.x + threadIdx.x; n < N; n += gridDim.y * blockDim.x) {
+        int j = (index + c + drow[n]) % dbs;
+        size_t off = ((size_t)(j / count) * ndm + dm) * ((size_t)count * N) + (size_t)(j % count) * N + n;
+        float2 x = arow[n], h = brow[n];
+        float2 r = make_float2((x.x * h.x - x.y * h.y) * invN, (x.x * h.y + x.y * h.x) * invN);
+        ring[off] = r;
+    }
+}
+
+// gpu_kernel.cu:236-258 -- keep samples [M, 2M) of each inverse FFT, * 1/(2M)
+__global__ void k_discard(const float2 *__restrict__ a, float2 *__restrict__ b, int M, size_t groups)
+{
+    const float scale = 1.0f / (float)(2 * M);
+    const size_t total = groups * (size_t)M;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+        size_t g = i / M, o = i % M;
+        float2 v = a[g * 2 * M + M + o];
+        b[i] = make_float2(v.x * scale, v.y * scale);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Filter-table generation.
+//
+// Chirp C(w_i) in IDFT input order (msosm_prepare.cpp:50-63): the phase
+// 4 pi^2 kdm dm w^2 / ((w + w0) w0^2) is evaluated in FP64.  mode 0 follows
+// the reference's float-rounded grid (w_i and w_i + w0 rounded to float as
+// its float arrays do); mode 1 uses an exact FP64 grid f = i*fs/M and
+// phi = 2 pi D DM f^2 / (f0^2 (f0 + f)).
+__global__ void k_chirp(float2 *__restrict__ c, unsigned long M, int mode, const ChirpParams *__restrict__ prm)
+{
+#pragma clang fp contract(off)
+    const double PI = 3.14159265358979323846;
+    const int d = blockIdx.y;
+    const ChirpParams p = prm[d];
+    float2 *cd = c + (size_t)d * M;
+    for (unsigned long i = blockIdx.x * (unsigned long)blockDim.x + threadIdx.x; i < M;
+         i += (unsigned long)gridDim.x * blockDim.x) {
+        double ph;
+        if (mode == 0) {
+            float sw = p.step * (float)i;                       // float product
+            float wi = (float)(-PI * (double)p.fs + (double)sw);
+            wi = (float)((double)wi + PI * (double)p.fs);
+            float wsum = wi + p.w0;                             // float sum
+            ph = 4 * PI;
+            ph = ph * PI;
+            ph = ph * (double)p.kdm;
+            ph = ph * (double)p.dm;
+            ph = ph * (double)wi;
+            ph = ph * (double)wi;
+            ph = ph / (double)wsum;
+            ph = ph / (double)p.w0;
+            ph = ph / (double)p.w0;
+        } else {
+            double f = (double)p.fs * (double)i / (double)M;
+            double F0 = (double)p.f0;
+            ph = 2 * PI * 4.15e15 * (double)p.dm * f * f / (F0 * F0 * (F0 + f));
+        }
+        double s, co;
+        sincos(ph, &s, &co);
+        unsigned long idx = (i < M / 2) ? (M / 2 + i) : (i - M / 2);
+        cd[idx] = make_float2((float)co, (float)(-s));
+    }
+}
+
+__global__ void k_zero_pad(float2 *__restrict__ dst, const float2 *__restrict__ src, unsigned long M)
+{
+    const int d = blockIdx.y;
+    for (unsigned long i = blockIdx.x * (unsigned long)blockDim.x + threadIdx.x; i < 2 * M;
+         i += (unsigned long)gridDim.x * blockDim.x)
+        dst[(size_t)d * 2 * M + i] = (i < M) ? src[(size_t)d * M + i] : make_float2(0.f, 0.f);
+}
+
+__global__ void k_to_half(__half2 *__restrict__ dst, const float2 *__restrict__ src, size_t n, float scale)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        float2 v = src[i];
+        dst[i] = __floats2half2_rn(v.x * scale, v.y * scale);
+    }
+}
+
+__global__ void k_i32_to_u16(uint16_t *__restrict__ dst, const int32_t *__restrict__ src, size_t n)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = (uint16_t)src[i];
+}
+
+// ---------------------------------------------------------------------------
+// Single-pass batched C2C: TPB transforms per workgroup, thread t of
+// transform tl handles points t + m*NT (consecutive t -> consecutive
+// addresses, so every load and store is coalesced).
+template <int LOG2N, bool INV>
+__global__ __launch_bounds__(fft_threads(LOG2N)) void k_fft_c2c(
+    const float2 *__restrict__ in, float2 *__restrict__ out, long long idist, long long odist, long long batch,
+    float scale)
+{
+    constexpr int E = fft_e(LOG2N);
+    using S = FftShape<LOG2N, E>;
+    constexpr int TPB = fft_tpb(LOG2N);
+    __shared__ float2 smem[TPB * S::NPAD + S::TWLO + S::TWHI];
+    float2 *twlo = smem + TPB * S::NPAD, *twhi = twlo + S::TWLO;
+    const int tl = threadIdx.x / S::NT, t = threadIdx.x % S::NT;
+    if constexpr (S::NPASS > 1)
+        fill_twiddles<LOG2N, E>(twlo, twhi, threadIdx.x, blockDim.x);
+    const long long b = (long long)blockIdx.x * TPB + tl;
+    const bool valid = b < batch;
+    float2 v[E];
+    const float2 *src = in + b * idist;
+#pragma unroll
+    for (int m = 0; m < E; m++)
+        v[m] = valid ? src[t + m * S::NT] : make_float2(0.f, 0.f);
+    fft_block<LOG2N, E, INV>(v, smem + tl * S::NPAD, twlo, twhi, t);
+    if (valid) {
+        float2 *dst = out + b * odist;
+#pragma unroll
+        for (int m = 0; m < E; m++)
+            dst[t + m * S::NT] = cscale(v[m], scale);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Two-pass (four-step) C2C for N = N1 * N2 > 2^14:
+//   pass A: for every column n2, DFT_N1 over x[N2*n1 + n2], times
+//           W_N^(n2*k1), stored to T[k1*N2 + n2];
+//   pass B: for every row k1, DFT_N2 over T[k1*N2 + n2], stored to
+//           X[k1 + N1*k2].
+// Pass A runs C columns per workgroup with the column index fastest across
+// lanes (coalesced strided access); pass B loads rows with the point index
+// fastest and transposes through LDS so its strided stores are coalesced.
+__host__ __device__ constexpr int cols_per_wg(int log2n)
+{
+    int nt = (1 << log2n) / fft_e(log2n);
+    int npad = (1 << log2n) + ((1 << log2n) >> ilog2c(fft_e(log2n)));
+    int c = 64;
+    while (c > 1 && (c * nt > 1024 || c * npad * 8 > 140 * 1024))
+        c >>= 1;
+    return c;
+}
+
+__host__ __device__ constexpr int cols_threads(int log2n) { return ((1 << log2n) / fft_e(log2n)) * cols_per_wg(log2n); }
+
+template <int LOG2N, bool INV>
+__global__ __launch_bounds__(cols_threads(LOG2N)) void k_fft_passA(
+    const float2 *__restrict__ in, float2 *__restrict__ out, long long idist, long long batch, int log2n2,
+    int log2ntot)
+{
+    constexpr int E = fft_e(LOG2N);
+    using S = FftShape<LOG2N, E>;
+    constexpr int C = cols_per_wg(LOG2N);
+    __shared__ float2 smem[C * S::NPAD + S::TWLO + S::TWHI];
+    float2 *twlo = smem + C * S::NPAD, *twhi = twlo + S::TWLO;
+    if constexpr (S::NPASS > 1)
+        fill_twiddles<LOG2N, E>(twlo, twhi, threadIdx.x, blockDim.x);
+    const int c = threadIdx.x % C, t = threadIdx.x / C;
+    const long long N2 = 1ll << log2n2;
+    const long long ncols = batch * N2;
+    const long long col = (long long)blockIdx.x * C + c;
+    const bool valid = col < ncols;
+    const long long b = col >> log2n2, n2 = col & (N2 - 1);
+    float2 v[E];
+    const float2 *src = in + b * idist + n2;
+#pragma unroll
+    for (int m = 0; m < E; m++)
+        v[m] = valid ? src[(long long)(t + m * S::NT) * N2] : make_float2(0.f, 0.f);
+    fft_block<LOG2N, E, INV>(v, smem + c * S::NPAD, twlo, twhi, t);
+    if (valid) {
+        const long long ntot = 1ll << log2ntot;
+        float2 *dst = out + (b << log2ntot) + n2;
+#pragma unroll
+        for (int m = 0; m < E; m++) {
+            long long k1 = t + m * S::NT;
+            long long e = (n2 * k1) & (ntot - 1);
+            double s, co;
+            sincospi(2.0 * (double)e / (double)ntot, &s, &co);
+            float2 w = make_float2((float)co, INV ? (float)s : (float)(-s));
+            dst[k1 * N2] = cmul(v[m], w);
+        }
+    }
+}
+
+template <int LOG2N, bool INV>
+__global__ __launch_bounds__(cols_threads(LOG2N)) void k_fft_passB(
+    const float2 *__restrict__ in, float2 *__restrict__ out, long long odist, long long batch, int log2n1,
+    float scale)
+{
+    constexpr int E = fft_e(LOG2N);
+    using S = FftShape<LOG2N, E>;
+    constexpr int C = cols_per_wg(LOG2N);
+    __shared__ float2 smem[C * S::NPAD + S::TWLO + S::TWHI];
+    float2 *twlo = smem + C * S::NPAD, *twhi = twlo + S::TWLO;
+    if constexpr (S::NPASS > 1)
+        fill_twiddles<LOG2N, E>(twlo, twhi, threadIdx.x, blockDim.x);
+    const long long N1 = 1ll << log2n1;
+    const long long nrows = batch * N1;
+    // load phase: transform tl, point index fastest
+    {
+        const int tl = threadIdx.x / S::NT, t = threadIdx.x % S::NT;
+        const long long row = (long long)blockIdx.x * C + tl;
+        const bool valid = row < nrows;
+        float2 v[E];
+        const float2 *src = in + row * S::N;
+#pragma unroll
+        for (int m = 0; m < E; m++)
+            v[m] = valid ? src[t + m * S::NT] : make_float2(0.f, 0.f);
+        fft_block<LOG2N, E, INV>(v, smem + tl * S::NPAD, twlo, twhi, t);
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < E; m++)
+            smem[tl * S::NPAD + lidx<S::LOGE>(t + m * S::NT)] = v[m];
+    }
+    __syncthreads();
+    // store phase: column (row index) fastest across lanes
+    {
+        const int c = threadIdx.x % C, t = threadIdx.x / C;
+        const long long row = (long long)blockIdx.x * C + c;
+        if (row < nrows) {
+            const long long b = row >> log2n1, k1 = row & (N1 - 1);
+            float2 *dst = out + b * odist + k1;
+#pragma unroll
+            for (int m = 0; m < E; m++) {
+                int k2 = t + m * S::NT;
+                dst[(long long)k2 * N1] = cscale(smem[c * S::NPAD + lidx<S::LOGE>(k2)], scale);
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Fused forward: window w of this call covers stream samples
+// [(w-1)M, (w+1)M) relative to the call start; samples before the call come
+// from the previous call's tail.  The spectrum goes to ring slot
+// (g0 + w) mod rx, either FP32 or FP16 scaled by 1/N (keeps |X| in range:
+// |x| <= 32768*sqrt(2) bounds |X|/N by 46341 < 65504).
+template <int LOG2N, bool F16>
+__global__ __launch_bounds__(fft_threads(LOG2N)) void k_fwd_u16(
+    const uint32_t *__restrict__ cur, const uint32_t *__restrict__ tail, void *__restrict__ ring, int count,
+    long long g0, int rx)
+{
+    constexpr int E = fft_e(LOG2N);
+    using S = FftShape<LOG2N, E>;
+    constexpr int TPB = fft_tpb(LOG2N);
+    constexpr long long N = S::N, M = S::N / 2;
+    __shared__ float2 smem[TPB * S::NPAD + S::TWLO + S::TWHI];
+    float2 *twlo = smem + TPB * S::NPAD, *twhi = twlo + S::TWLO;
+    const int tl = threadIdx.x / S::NT, t = threadIdx.x % S::NT;
+    if constexpr (S::NPASS > 1)
+        fill_twiddles<LOG2N, E>(twlo, twhi, threadIdx.x, blockDim.x);
+    const int w = blockIdx.x * TPB + tl;
+    const bool valid = w < count;
+    float2 v[E];
+#pragma unroll
+    for (int m = 0; m < E; m++) {
+        long long rel = (long long)(w - 1) * M + t + m * S::NT;
+        uint32_t u = 0;
+        if (valid)
+            u = rel < 0 ? tail[M + rel] : cur[rel];
+        v[m] = unpack_u16pair(u);
+    }
+    fft_block<LOG2N, E, false>(v, smem + tl * S::NPAD, twlo, twhi, t);
+    if (valid) {
+        const long long slot = (g0 + w) % rx;
+        if constexpr (F16) {
+            __half2 *dst = reinterpret_cast<__half2 *>(ring) + slot * N;
+            constexpr float invN = 1.0f / (float)N;
+#pragma unroll
+            for (int m = 0; m < E; m++)
+                dst[t + m * S::NT] = __floats2half2_rn(v[m].x * invN, v[m].y * invN);
+        } else {
+            float2 *dst = reinterpret_cast<float2 *>(ring) + slot * N;
+#pragma unroll
+            for (int m = 0; m < E; m++)
+                dst[t + m * S::NT] = v[m];
+        }
+    }
+}
+
+// Fused inverse: unit u = (dm, block b) with blocks of one DM consecutive.
+// Y[n] = X_{g-d(n)}[n] * H[n] (zero before the stream start), inverse FFT,
+// keep points [M, 2M), scale: FP32 ring/H -> 1/N^2 (the reference's two 1/N
+// factors, gpu_kernel.cu:169,251), FP16 ring (X/N) and H (H/M) -> 1/2.
+template <int LOG2N, bool F16>
+__global__ __launch_bounds__(fft_threads(LOG2N)) void k_inv_fused(
+    const void *__restrict__ ring, const void *__restrict__ H, const uint16_t *__restrict__ delay,
+    float2 *__restrict__ out, int count, int ndm, long long g0, int rx)
+{
+    constexpr int E = fft_e(LOG2N);
+    using S = FftShape<LOG2N, E>;
+    constexpr int TPB = fft_tpb(LOG2N);
+    constexpr long long N = S::N, M = S::N / 2;
+    __shared__ float2 smem[TPB * S::NPAD + S::TWLO + S::TWHI];
+    float2 *twlo = smem + TPB * S::NPAD, *twhi = twlo + S::TWLO;
+    const int tl = threadIdx.x / S::NT, t = threadIdx.x % S::NT;
+    if constexpr (S::NPASS > 1)
+        fill_twiddles<LOG2N, E>(twlo, twhi, threadIdx.x, blockDim.x);
+    const long long u = (long long)blockIdx.x * TPB + tl;
+    const long long units = (long long)ndm * count;
+    const bool valid = u < units;
+    const int dm = valid ? (int)(u / count) : 0;
+    const int b = valid ? (int)(u % count) : 0;
+    const long long g = g0 + b;
+    const uint16_t *drow = delay + (long long)dm * N;
+    float2 v[E];
+#pragma unroll
+    for (int m = 0; m < E; m++) {
+        const int n = t + m * S::NT;
+        float2 x = make_float2(0.f, 0.f);
+        if (valid) {
+            long long src = g - (long long)drow[n];
+            if (src >= 0) {
+                long long off = (src % rx) * N + n;
+                if constexpr (F16) {
+                    float2 xs = __half22float2(reinterpret_cast<const __half2 *>(ring)[off]);
+                    float2 h = __half22float2(reinterpret_cast<const __half2 *>(H)[(long long)dm * N + n]);
+                    x = cmul(xs, h);
+                } else {
+                    float2 xs = reinterpret_cast<const float2 *>(ring)[off];
+                    float2 h = reinterpret_cast<const float2 *>(H)[(long long)dm * N + n];
+                    x = cmul(xs, h);
+                }
+            }
+        }
+        v[m] = x;
+    }
+    fft_block<LOG2N, E, true>(v, smem + tl * S::NPAD, twlo, twhi, t);
+    if (valid) {
+        const float scale = F16 ? 0.5f : 1.0f / ((float)N * (float)N);
+        float2 *dst = out + ((long long)dm * count + b) * M;
+#pragma unroll
+        for (int m = E / 2; m < E; m++)
+            dst[t + m * S::NT - M] = cscale(v[m], scale);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Launchers
+
+#define MSOSM_LAUNCH_CHECK()                                                                                     \
+    do {                                                                                                         \
+        hipError_t e__ = hipGetLastError();                                                                      \
+        if (e__ != hipSuccess)                                                                                   \
+            return e__;                                                                                          \
+    } while (0)
+
+template <int LOG2N>
+static hipError_t fft_single(const float2 *in, float2 *out, long long batch, long long idist, long long odist,
+                             bool inv, float scale, hipStream_t s)
+{
+    constexpr int E = fft_e(LOG2N);
+    constexpr int threads = FftShape<LOG2N, E>::NT * fft_tpb(LOG2N);
+    const long long grid = (batch + fft_tpb(LOG2N) - 1) / fft_tpb(LOG2N);
+    if (inv)
+        hipLaunchKernelGGL((k_fft_c2c<LOG2N, true>), dim3((unsigned)grid), dim3(threads), 0, s, in, out, idist,
+                           odist, batch, scale);
+    else
+        hipLaunchKernelGGL((k_fft_c2c<LOG2N, false>), dim3((unsigned)grid), dim3(threads), 0, s, in, out, idist,
+                           odist, batch, scale);
+    MSOSM_LAUNCH_CHECK();
+    return hipSuccess;
+}
+
+template <int LOG2N>
+static hipError_t fft_passA(const float2 *in, float2 *out, long long idist, long long batch, int log2n2,
+                            int log2ntot, bool inv, hipStream_t s)
+{
+    constexpr int E = fft_e(LOG2N);
+    constexpr int C = cols_per_wg(LOG2N);
+    constexpr int threads = FftShape<LOG2N, E>::NT * C;
+    const long long ncols = batch << log2n2;
+    const long long grid = (ncols + C - 1) / C;
+    if (inv)
+        hipLaunchKernelGGL((k_fft_passA<LOG2N, true>), dim3((unsigned)grid), dim3(threads), 0, s, in, out, idist,
+                           batch, log2n2, log2ntot);
+    else
+        hipLaunchKernelGGL((k_fft_passA<LOG2N, false>), dim3((unsigned)grid), dim3(threads), 0, s, in, out, idist,
+                           batch, log2n2, log2ntot);
+    MSOSM_LAUNCH_CHECK();
+    return hipSuccess;
+}
+
+template <int LOG2N>
+static hipError_t fft_passB(const float2 *in, float2 *out, long long odist, long long batch, int log2n1, bool inv,
+                            float scale, hipStream_t s)
+{
+    constexpr int E = fft_e(LOG2N);
+    constexpr int C = cols_per_wg(LOG2N);
+    constexpr int threads = FftShape<LOG2N, E>::NT * C;
+    const long long nrows = batch << log2n1;
+    const long long grid = (nrows + C - 1) / C;
+    if (inv)
+        hipLaunchKernelGGL((k_fft_passB<LOG2N, true>), dim3((unsigned)grid), dim3(threads), 0, s, in, out, odist,
+                           batch, log2n1, scale);
+    else
+        hipLaunchKernelGGL((k_fft_passB<LOG2N, false>), dim3((unsigned)grid), dim3(threads), 0, s, in, out, odist,
+                           batch, log2n1, scale);
+    MSOSM_LAUNCH_CHECK();
+    return hipSuccess;
+}
+
+#define MSOSM_DISPATCH_1_14(L, CALL)                                                                             \
+    switch (L) {                                                                                                 \
+    case 1: return CALL(1);                                                                                      \
+    case 2: return CALL(2);                                                                                      \
+    case 3: return CALL(3);                                                                                      \
+    case 4: return CALL(4);                                                                                      \
+    case 5: return CALL(5);                                                                                      \
+    case 6: return CALL(6);                                                                                      \
+    case 7: return CALL(7);                                                                                      \
+    case 8: return CALL(8);                                                                                      \
+    case 9: return CALL(9);                                                                                      \
+    case 10: return CALL(10);                                                                                    \
+    case 11: return CALL(11);                                                                                    \
+    case 12: return CALL(12);                                                                                    \
+    case 13: return CALL(13);                                                                                    \
+    case 14: return CALL(14);                                                                                    \
+    default: return hipErrorInvalidValue;                                                                        \
+    }
+
+static void split_two_pass(int log2n, int &a, int &b)
+{
+    a = log2n - kMaxLog2SinglePass;
+    if (a < log2n / 2)
+        a = log2n / 2;
+    b = log2n - a;
+}
+
+size_t fft_c2c_scratch_elems(int log2n, long long batch)
+{
+    if (log2n <= kMaxLog2SinglePass)
+        return 0;
+    return (size_t)batch << log2n;
+}
+
+hipError_t launch_fft_c2c(const float2 *in, float2 *out, int log2n, long long batch, long long idist,
+                          long long odist, bool inverse, float scale, float2 *scratch, hipStream_t s)
+{
+    if (log2n < 1 || log2n > 28 || batch < 1)
+        return hipErrorInvalidValue;
+    if (log2n <= kMaxLog2SinglePass) {
+#define CALL_SINGLE(L) fft_single<L>(in, out, batch, idist, odist, inverse, scale, s)
+        MSOSM_DISPATCH_1_14(log2n, CALL_SINGLE)
+#undef CALL_SINGLE
+    }
+    int a, b;
+    split_two_pass(log2n, a, b);
+    bool own = false;
+    if (!scratch) {
+        hipError_t e = hipMallocAsync((void **)&scratch, fft_c2c_scratch_elems(log2n, batch) * sizeof(float2), s);
+        if (e != hipSuccess)
+            return e;
+        own = true;
+    }
+    hipError_t e;
+    {
+#define CALL_A(L) fft_passA<L>(in, scratch, idist, batch, b, log2n, inverse, s)
+        e = [&]() -> hipError_t { MSOSM_DISPATCH_1_14(a, CALL_A) }();
+#undef CALL_A
+    }
+    if (e == hipSuccess) {
+#define CALL_B(L) fft_passB<L>(scratch, out, odist, batch, a, inverse, scale, s)
+        e = [&]() -> hipError_t { MSOSM_DISPATCH_1_14(b, CALL_B) }();
+#undef CALL_B
+    }
+    if (own) {
+        hipError_t e2 = hipFreeAsync(scratch, s);
+        if (e == hipSuccess)
+            e = e2;
+    }
+    return e;
+}
+
+bool fused_supported(int log2n) { return log2n >= 2 && log2n <= kMaxLog2SinglePass; }
+
+template <int LOG2N>
+static hipError_t fwd_u16(const uint32_t *cur, const uint32_t *tail, void *ring, bool f16, int count, long long g0,
+                          int rx, hipStream_t s)
+{
+    constexpr int E = fft_e(LOG2N);
+    constexpr int threads = FftShape<LOG2N, E>::NT * fft_tpb(LOG2N);
+    const int grid = (count + fft_tpb(LOG2N) - 1) / fft_tpb(LOG2N);
+    if (f16)
+        hipLaunchKernelGGL((k_fwd_u16<LOG2N, true>), dim3(grid), dim3(threads), 0, s, cur, tail, ring, count, g0,
+                           rx);
+    else
+        hipLaunchKernelGGL((k_fwd_u16<LOG2N, false>), dim3(grid), dim3(threads), 0, s, cur, tail, ring, count, g0,
+                           rx);
+    MSOSM_LAUNCH_CHECK();
+    return hipSuccess;
+}
+
+template <int LOG2N>
+static hipError_t inv_fused(const void *ring, const void *H, const uint16_t *delay, float2 *out, bool f16,
+                            int count, int ndm, long long g0, int rx, hipStream_t s)
+{
+    constexpr int E = fft_e(LOG2N);
+    constexpr int threads = FftShape<LOG2N, E>::NT * fft_tpb(LOG2N);
+    const long long units = (long long)count * ndm;
+    const long long grid = (units + fft_tpb(LOG2N) - 1) / fft_tpb(LOG2N);
+    if (f16)
+        hipLaunchKernelGGL((k_inv_fused<LOG2N, true>), dim3((unsigned)grid), dim3(threads), 0, s, ring, H, delay,
+                           out, count, ndm, g0, rx);
+    else
+        hipLaunchKernelGGL((k_inv_fused<LOG2N, false>), dim3((unsigned)grid), dim3(threads), 0, s, ring, H, delay,
+                           out, count, ndm, g0, rx);
+    MSOSM_LAUNCH_CHECK();
+    return hipSuccess;
+}
+
+#define MSOSM_DISPATCH_2_14(L, CALL)                                                                             \
+    switch (L) {                                                                                                 \
+    case 2: return CALL(2);                                                                                      \
+    case 3: return CALL(3);                                                                                      \
+    case 4: return CALL(4);                                                                                      \
+    case 5: return CALL(5);                                                                                      \
+    case 6: return CALL(6);                                                                                      \
+    case 7: return CALL(7);                                                                                      \
+    case 8: return CALL(8);                                                                                      \
+    case 9: return CALL(9);                                                                                      \
+    case 10: return CALL(10);                                                                                    \
+    case 11: return CALL(11);                                                                                    \
+    case 12: return CALL(12);                                                                                    \
+    case 13: return CALL(13);                                                                                    \
+    case 14: return CALL(14);                                                                                    \
+    default: return hipErrorInvalidValue;                                                                        \
+    }
+
+hipError_t launch_fwd_u16(const uint32_t *cur, const uint32_t *tail, void *ring, bool f16, int log2n, int count,
+                          long long g0, int rx, hipStream_t s)
+{
+#define CALL_F(L) fwd_u16<L>(cur, tail, ring, f16, count, g0, rx, s)
+    MSOSM_DISPATCH_2_14(log2n, CALL_F)
+#undef CALL_F
+}
+
+hipError_t launch_inv_fused(const void *ring, const void *H, const uint16_t *delay, float2 *out, bool f16,
+                            int log2n, int count, int ndm, long long g0, int rx, hipStream_t s)
+{
+#define CALL_I(L) inv_fused<L>(ring, H, delay, out, f16, count, ndm, g0, rx, s)
+    MSOSM_DISPATCH_2_14(log2n, CALL_I)
+#undef CALL_I
+}
+
+hipError_t launch_unpack(float2 *dst, const uint32_t *src, size_t n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_unpack, dim3(grid_for(n)), dim3(kBlock), 0, s, dst, src, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack(uint32_t *dst, const float2 *src, size_t n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_pack, dim3(grid_for(n)), dim3(kBlock), 0, s, dst, src, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_intensity(const float2 *a, const float2 *b, float *out, size_t n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_intensity, dim3(grid_for(n)), dim3(kBlock), 0, s, a, b, out, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_multiply(const float2 *a, const float2 *b, int N, float2 *out, int count, int ndm, hipStream_t s)
+{
+    size_t total = (size_t)N * count * ndm;
+    hipLaunchKernelGGL(k_multiply, dim3(grid_for(total)), dim3(kBlock), 0, s, a, b, out, N, count, ndm);
+    return hipGetLastError();
+}
+
+hipError_t launch_multiply_delay(const float2 *a, const float2 *b, int N, float2 *ring, const int32_t *delay,
+                                 int dbs, int index, int count, int ndm, hipStream_t s)
+{
+    int gy = (N + kBlock - 1) / kBlock;
+    if (gy > 64)
+        gy = 64;
+    hipLaunchKernelGGL(k_multiply_delay, dim3((unsigned)((long long)count * ndm), gy), dim3(kBlock), 0, s, a, b,
+                       ring, delay, dbs, index, N, count, ndm);
+    return hipGetLastError();
+}
+
+hipError_t launch_discard(const float2 *a, float2 *b, int M, int count, int ndm, hipStream_t s)
+{
+    size_t groups = (size_t)count * ndm;
+    hipLaunchKernelGGL(k_discard, dim3(grid_for(groups * M)), dim3(kBlock), 0, s, a, b, M, groups);
+    return hipGetLastError();
+}
+
+hipError_t launch_chirp(float2 *c, unsigned long M, int mode, const ChirpParams *params_dev, int ndm,
+                        hipStream_t s)
+{
+    int gx = grid_for(M);
+    if (gx > 4096)
+        gx = 4096;
+    hipLaunchKernelGGL(k_chirp, dim3(gx, ndm), dim3(kBlock), 0, s, c, M, mode, params_dev);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero_pad(float2 *dst, const float2 *src, unsigned long M, int ndm, hipStream_t s)
+{
+    int gx = grid_for(2 * M);
+    if (gx > 4096)
+        gx = 4096;
+    hipLaunchKernelGGL(k_zero_pad, dim3(gx, ndm), dim3(kBlock), 0, s, dst, src, M);
+    return hipGetLastError();
+}
+
+hipError_t launch_to_half(__half2 *dst, const float2 *src, size_t n, float scale, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_to_half, dim3(grid_for(n)), dim3(kBlock), 0, s, dst, src, n, scale);
+    return hipGetLastError();
+}
+
+hipError_t launch_i32_to_u16(uint16_t *dst, const int32_t *src, size_t n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_i32_to_u16, dim3(grid_for(n)), dim3(kBlock), 0, s, dst, src, n);
+    return hipGetLastError();
+}
+
+} // namespace msosm

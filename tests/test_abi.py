@@ -1,0 +1,53 @@
+"""The C ABI library loads and exports every function include/msosm_c.h
+declares (no GPU calls)."""
+import ctypes
+import os
+import re
+import subprocess
+
+import msosm_gpu_amd as ms
+
+HEADER = os.path.join(ms.INCLUDE_DIR, "msosm_c.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(msosm_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_loads_and_exports_declared_symbols():
+    L = ms.lib()
+    names = declared_functions()
+    assert len(names) >= 30
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    # the python binding covers the whole header
+    assert set(names) <= set(ms.SIGNATURES), set(names) - set(ms.SIGNATURES)
+    out = subprocess.run(["nm", "-D", "--defined-only", ms.LIB_PATH], capture_output=True, text=True).stdout
+    exported = set(re.findall(r" T (msosm_\w+)", out))
+    assert set(names) <= exported
+
+
+def test_version_string():
+    assert "gfx950" in ms.version()
+
+
+def test_library_contains_gfx950_code():
+    # the offload bundle inside the shared object names its target
+    data = open(ms.LIB_PATH, "rb").read()
+    assert b"amdgcn-amd-amdhsa--gfx950" in data
+
+
+def test_no_gpu_call_needed_for_plans():
+    # planning works without a GPU; engine creation reports a clean error when none is present
+    p = ms.Plan(16e6, 10.0, 1392e6, fftpoint=1024, count=4)
+    try:
+        import torch
+        has_gpu = torch.cuda.is_available()
+    except Exception:
+        has_gpu = False
+    if not has_gpu:
+        h = ctypes.c_void_p()
+        rc = ms.lib().msosm_engine_create(p.h, None, ctypes.byref(h))
+        assert rc != 0 and h.value is None

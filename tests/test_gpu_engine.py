@@ -1,0 +1,215 @@
+"""GPU parity of the dedispersion engine (all execution forms) against the
+FP64 same-plan oracle.  Gates: FP32 forms <= 1e-5 relative L2, the
+FP16-storage form <= 1e-3 relative L2 (per DM, whole stream including the
+zero-filled transient)."""
+import numpy as np
+import pytest
+
+import msosm_gpu_amd as ms
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+FP32_TOL = 1e-5
+MIXED_TOL = 1e-3
+FORMS = [ms.FORM_PLAIN_F32, ms.FORM_FUSED_F32, ms.FORM_FUSED_F16]
+TOL = {ms.FORM_PLAIN_F32: FP32_TOL, ms.FORM_FUSED_F32: FP32_TOL, ms.FORM_FUSED_F16: MIXED_TOL}
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(np.asarray(b)), 1e-300))
+
+
+def make_input(L, seed, sigma=900.0, zeros=32):
+    rng = np.random.Generator(np.random.Philox(seed))
+    u = np.clip(np.round(32768 + sigma * rng.standard_normal(2 * L)), 1, 65535).astype(np.uint16)
+    if zeros:
+        u[rng.integers(0, 2 * L, zeros)] = 0
+    return u
+
+
+def run_engine(plan, u, device_input=True, device=None):
+    """Feed u (2L uint16) call by call; returns output [ndm, L] complex64."""
+    import torch
+    eng = ms.Engine(plan)
+    n = plan.count * plan.M
+    L = u.size // 2
+    assert L % n == 0
+    outs = []
+    words = u.view(np.uint32)
+    dev = torch.from_numpy(words.view(np.int32).copy()).to(device) if device_input else None
+    for c in range(L // n):
+        if device_input:
+            eng.filter_u16(dev[c * n:(c + 1) * n])
+        else:
+            eng.filter_u16(words[c * n:(c + 1) * n])
+        outs.append(eng.output_c64())
+    eng.close()
+    y = np.concatenate(outs, axis=1)  # [ndm, calls*count, M]
+    return y.reshape(plan.num_dms, -1)
+
+
+def oracle_out(bw, dm, f0, M, u, mode=oracle.CHIRP_REF_F32W, method=ms.METHOD_MSOSM):
+    x = oracle.unpack_u16(u).astype(np.complex128)
+    if method == ms.METHOD_MSOSM:
+        d, K, _ = oracle.delay_points(bw, dm, f0, M)
+    else:
+        d = np.zeros(2 * M, np.int32)
+    H = oracle.dedisp_params(bw, dm, f0, M, mode)
+    return oracle.msosm_run(x, M, d, H)
+
+
+@pytest.mark.parametrize("form", FORMS)
+def test_golden_pair_config1_plan(device, golden, form):
+    arrays, meta = golden
+    c = meta["msosm"]
+    u = arrays["msosm_in_u16"]
+    p = ms.Plan(c["bw"], c["dm"], c["f0"], fftpoint=2 * c["M"], count=8, form=form)
+    print(f"golden pair form={form}", flush=True)
+    y = run_engine(p, u, device=device)[0]
+    assert rel_l2(y, arrays["msosm_out_ref"]) < TOL[form]
+
+
+@pytest.mark.parametrize("form", FORMS)
+@pytest.mark.parametrize("mode", [ms.CHIRP_REF_F32W, ms.CHIRP_EXACT_F64])
+def test_filter_tables_match_oracle(device, form, mode):
+    bw, dm, f0 = 16e6, 10.0, 1392e6
+    p = ms.Plan(bw, dm, f0, fftpoint=1024, count=2, form=form, chirp=mode)
+    eng = ms.Engine(p)
+    H = eng.dedisp_params(0)
+    eng.close()
+    assert rel_l2(H, oracle.dedisp_params(bw, dm, f0, 512, mode)) < 1e-6
+
+
+@pytest.mark.parametrize("form", FORMS)
+def test_config1_single_dm_vs_fp64(device, form):
+    # 16 MHz, DM 10, f0 1392 MHz, N = 1024 (M = 512, K = 16), 2^18 samples
+    bw, dm, f0 = 16e6, 10.0, 1392e6
+    u = make_input(1 << 18, 0x5EED0001)
+    p = ms.Plan(bw, dm, f0, fftpoint=1024, count=32, form=form)
+    assert (p.M, p.K_max, p.R) == (512, 16, 64)
+    print(f"config1 form={form}", flush=True)
+    y = run_engine(p, u, device=device)[0]
+    ref = oracle_out(bw, dm, f0, 512, u)
+    err = rel_l2(y, ref)
+    print(f"  rel L2 = {err:.3e}")
+    assert err < TOL[form]
+
+
+@pytest.mark.parametrize("form", FORMS)
+def test_exact_chirp_mode(device, form):
+    bw, dm, f0 = 32e6, 60.0, 1384e6
+    u = make_input(1 << 17, 7)
+    p = ms.Plan(bw, dm, f0, fftpoint=4096, count=4, form=form, chirp=ms.CHIRP_EXACT_F64)
+    y = run_engine(p, u, device=device)[0]
+    ref = oracle_out(bw, dm, f0, 2048, u, mode=oracle.CHIRP_EXACT_F64)
+    assert rel_l2(y, ref) < TOL[form]
+
+
+@pytest.mark.parametrize("form", FORMS)
+def test_multi_dm_vs_fp64(device, form):
+    # 8 DM trials on one 16384-point plan, K from 1 (DM 0) to 44
+    bw, f0 = 64e6, 1368e6
+    dms = [0.0, 0.4, 3.0, 7.5, 12.0, 18.25, 24.0, 29.0]
+    p = ms.Plan(bw, dms, f0, fftpoint=16384, count=4, form=form)
+    u = make_input(16 * 8192, 99)
+    print(f"multi-dm form={form} K_max={p.K_max}", flush=True)
+    y = run_engine(p, u, device=device)
+    for i, dm in enumerate(dms):
+        ref = oracle_out(bw, dm, f0, 8192, u)
+        err = rel_l2(y[i], ref)
+        assert err < TOL[form], (i, dm, err)
+
+
+def test_block_split_invariance(device):
+    # 1 call of 16 blocks == 16 calls of 1 block == 4 calls of 4 (bit for bit)
+    bw, dm, f0 = 16e6, 10.0, 1392e6
+    u = make_input(64 * 512, 3)
+    for form in FORMS:
+        outs = []
+        for count in (16, 1, 4):
+            p = ms.Plan(bw, dm, f0, fftpoint=1024, count=count, form=form)
+            outs.append(run_engine(p, u, device=device))
+        np.testing.assert_array_equal(outs[0], outs[1])
+        np.testing.assert_array_equal(outs[0], outs[2])
+
+
+def test_host_and_device_input_agree(device):
+    bw, dm, f0 = 16e6, 10.0, 1392e6
+    u = make_input(32 * 512, 4)
+    p = ms.Plan(bw, dm, f0, fftpoint=1024, count=8, form=ms.FORM_FUSED_F16)
+    a = run_engine(p, u, device_input=True, device=device)
+    b = run_engine(p, u, device_input=False, device=device)
+    np.testing.assert_array_equal(a, b)
+
+
+def test_forms_agree(device):
+    bw, f0 = 8e6, 1396e6
+    dms = [50.0, 300.0]
+    u = make_input(1 << 16, 12)
+    outs = {f: run_engine(ms.Plan(bw, dms, f0, fftpoint=1024, count=8, form=f), u, device=device) for f in FORMS}
+    for i in range(2):
+        assert rel_l2(outs[ms.FORM_FUSED_F32][i], outs[ms.FORM_PLAIN_F32][i]) < 2e-6
+        assert rel_l2(outs[ms.FORM_FUSED_F16][i], outs[ms.FORM_PLAIN_F32][i]) < MIXED_TOL
+
+
+def test_osm_two_pass_fft(device):
+    # OSM plan of the 16 MHz / DM 10 case: M = 16384, FFT 32768 (two-pass)
+    bw, dm, f0 = 16e6, 10.0, 1392e6
+    p = ms.Plan(bw, dm, f0, fftpoint=0, count=2, method=ms.METHOD_OSM)
+    assert p.M == 16384 and p.K_max == 1
+    u = make_input(4 * 16384, 21)
+    y = run_engine(p, u, device=device)[0]
+    ref = oracle_out(bw, dm, f0, 16384, u, method=ms.METHOD_OSM)
+    assert rel_l2(y, ref) < FP32_TOL
+
+
+def test_config2_fft_size_plain(device):
+    # N = 2^16 (the 128 MHz production plan's FFT size) at DM 5: two-pass path
+    bw, dm, f0 = 128e6, 5.0, 1336e6
+    p = ms.Plan(bw, dm, f0, fftpoint=65536, count=2)
+    u = make_input(8 * 32768, 22)
+    y = run_engine(p, u, device=device)[0]
+    ref = oracle_out(bw, dm, f0, 32768, u)
+    assert rel_l2(y, ref) < FP32_TOL
+
+
+def test_output_u16_matches_pack(device):
+    bw, dm, f0 = 16e6, 10.0, 1392e6
+    u = make_input(8 * 512, 5)
+    p = ms.Plan(bw, dm, f0, fftpoint=1024, count=8)
+    eng = ms.Engine(p)
+    eng.filter_u16(u.view(np.uint32))
+    c = eng.output_c64()
+    q = eng.output_u16()
+    eng.close()
+    np.testing.assert_array_equal(q.reshape(-1), oracle.pack_u16(c.reshape(-1)))
+
+
+def test_reset_restarts_stream(device):
+    bw, dm, f0 = 16e6, 10.0, 1392e6
+    u = make_input(16 * 512, 6)
+    for form in FORMS:
+        p = ms.Plan(bw, dm, f0, fftpoint=1024, count=16, form=form)
+        eng = ms.Engine(p)
+        eng.filter_u16(u.view(np.uint32))
+        a = eng.output_c64()
+        eng.filter_u16(u.view(np.uint32))
+        eng.reset()
+        eng.filter_u16(u.view(np.uint32))
+        b = eng.output_c64()
+        eng.close()
+        np.testing.assert_array_equal(a, b)
+
+
+def test_timing_reports_stages(device):
+    p = ms.Plan(16e6, 10.0, 1392e6, fftpoint=1024, count=8, form=ms.FORM_PLAIN_F32)
+    eng = ms.Engine(p)
+    eng.set_timing(True)
+    u = make_input(8 * 512, 8)
+    for _ in range(3):
+        eng.filter_u16(u.view(np.uint32))
+    t = eng.get_timing()
+    eng.close()
+    assert t["fft_fwd"][1] == 3 and t["fft_inv"][1] == 3 and t["multiply"][1] == 3
+    assert all(v[0] >= 0 for v in t.values())
