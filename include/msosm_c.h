@@ -85,6 +85,10 @@ typedef struct msosm_fold msosm_fold;
  * overlap_save/osm_prepare.cpp:3-30) and the fftpoint/count logic of
  * initialize_uint16 (msosm_gpu_dm_concurrent.cpp:28-42). */
 int msosm_plan_create(const msosm_plan_desc *desc, msosm_plan **out);
+/* Scalars of one DM trial without a plan: Nd (Prepare_MSOSM ctor,
+ * msosm_prepare.cpp:8) and the minimum order M_min of the method
+ * (calculate_min_order, msosm_prepare.cpp:17-30 / osm_prepare.cpp:17-30). */
+int msosm_dm_model(float bw, float dm, float f0, int method, unsigned long *nd, unsigned long *m_min);
 void msosm_plan_destroy(msosm_plan *plan);
 int msosm_plan_get_info(const msosm_plan *plan, msosm_plan_info *info);
 /* Per-DM scalars: Nd, M_min, delaycount K, delay_block_size. */
@@ -179,6 +183,48 @@ int msosm_fold_intensity_ptr(msosm_fold *f, void **dev_ptr);
  * into the fold's buffer (the reference's public total_intensity array,
  * fold_gpu.h:31, which fold_data reads). */
 int msosm_fold_load_intensity(msosm_fold *f, const float *src, int where);
+
+/* Phase fold of engine output (north-star fold; no reference counterpart:
+ * the reference folds on the CPU with integer periods, fold_gpu.cpp:48-173).
+ * For each of ndm trials (rows of `stride` complex64 samples at a, and at b
+ * for a second polarisation or NULL), sample i with global index n0 + i goes
+ * to bin floor(frac((n0+i)/period_samples) * nbins), the phase computed in
+ * FP64 on the GPU; value sqrt(|a|^2+|b|^2) (power = 0, the reference's
+ * intensity) or |a|^2+|b|^2 (power = 1).  Adds into prof[ndm][nbins] and,
+ * when hits != NULL, the per-bin sample counts into hits[nbins] (device
+ * pointers; the caller zeroes them).  nbins <= 8192. */
+int msosm_fold_phase(const void *a, const void *b, size_t n, int ndm, size_t stride, long long n0,
+                     double period_samples, int nbins, int power, float *prof, float *hits, void *stream);
+
+/* Uneven multi-stream batch in one graph launch (north-star component; no
+ * reference counterpart: the reference runs one CUDA stream per engine with
+ * host launches per block, msosm_stream/msosm_gpu_batch_stream.cpp:105-149).
+ * engines[i] (distinct, created by the caller, timing off) each filter their
+ * own resident device input inputs[i] of num_samples[i] packed uint16 pairs
+ * (whole calls of count*M; a trailing partial call is ignored).  When
+ * period_samples != NULL, every call's output is phase-folded into nbins bins
+ * (power, FP64 phase) for streams with period_samples[i] > 0.  The whole
+ * pass -- engine resets, all calls, all folds, one graph branch per stream --
+ * is captured once into a hipGraph; msosm_batch_launch replays it. */
+typedef struct msosm_batch msosm_batch;
+int msosm_batch_create(msosm_engine *const *engines, int n, const void *const *inputs,
+                       const unsigned long *num_samples, const double *period_samples, int nbins, void *stream,
+                       msosm_batch **out);
+int msosm_batch_launch(msosm_batch *b);
+int msosm_batch_synchronize(msosm_batch *b);
+int msosm_batch_info(msosm_batch *b, int *num_streams, size_t *graph_nodes);
+/* Host copies of stream i's profiles [num_dms][nbins] and hit counts [nbins]. */
+int msosm_batch_profiles(msosm_batch *b, int i, float *prof, float *hits);
+int msosm_batch_destroy(msosm_batch *b);
+
+/* Autotuner (no reference counterpart): among powers of two in
+ * [min_fftpoint, max_fftpoint] that are not below the plan's minimum, time
+ * `calls` filter calls of about target_samples samples on synthetic input and
+ * return the FFT size with the highest DM-trial*sample throughput (results are
+ * cached per descriptor for the life of the process). */
+int msosm_autotune_fftpoint(const msosm_plan_desc *desc, unsigned long min_fftpoint, unsigned long max_fftpoint,
+                            unsigned long target_samples, int calls, void *stream, unsigned long *best_fftpoint,
+                            double *best_ns_per_dm_sample);
 
 const char *msosm_last_error(void);
 const char *msosm_version(void);

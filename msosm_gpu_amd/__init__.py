@@ -66,6 +66,7 @@ _f, _ll = ctypes.c_float, ctypes.c_longlong
 SIGNATURES = {
     "msosm_plan_create": (_i, [ctypes.POINTER(PlanDesc), ctypes.POINTER(_vp)]),
     "msosm_plan_destroy": (None, [_vp]),
+    "msosm_dm_model": (_i, [_f, _f, _f, _i, ctypes.POINTER(_ul), ctypes.POINTER(_ul)]),
     "msosm_plan_get_info": (_i, [_vp, ctypes.POINTER(PlanInfo)]),
     "msosm_plan_dm_info": (_i, [_vp, _i, ctypes.POINTER(_ul), ctypes.POINTER(_ul), ctypes.POINTER(_i),
                                 ctypes.POINTER(_i)]),
@@ -99,6 +100,16 @@ SIGNATURES = {
     "msosm_fold_info": (_i, [_vp, ctypes.POINTER(_ul), ctypes.POINTER(_ul), ctypes.POINTER(_i)]),
     "msosm_fold_intensity_ptr": (_i, [_vp, ctypes.POINTER(_vp)]),
     "msosm_fold_load_intensity": (_i, [_vp, _vp, _i]),
+    "msosm_fold_phase": (_i, [_vp, _vp, _sz, _i, _sz, _ll, ctypes.c_double, _i, _i, _vp, _vp, _vp]),
+    "msosm_batch_create": (_i, [ctypes.POINTER(_vp), _i, ctypes.POINTER(_vp), ctypes.POINTER(_ul),
+                                ctypes.POINTER(ctypes.c_double), _i, _vp, ctypes.POINTER(_vp)]),
+    "msosm_batch_launch": (_i, [_vp]),
+    "msosm_batch_synchronize": (_i, [_vp]),
+    "msosm_batch_info": (_i, [_vp, ctypes.POINTER(_i), ctypes.POINTER(_sz)]),
+    "msosm_batch_profiles": (_i, [_vp, _i, _vp, _vp]),
+    "msosm_batch_destroy": (_i, [_vp]),
+    "msosm_autotune_fftpoint": (_i, [ctypes.POINTER(PlanDesc), _ul, _ul, _ul, _i, _vp, ctypes.POINTER(_ul),
+                                     ctypes.POINTER(ctypes.c_double)]),
     "msosm_last_error": (ctypes.c_char_p, []),
     "msosm_version": (ctypes.c_char_p, []),
 }
@@ -369,6 +380,78 @@ def fft_c2c(inp, out, n, batch, idist=None, odist=None, direction=-1, stream=Non
     idist = n if idist is None else idist
     odist = n if odist is None else odist
     check(lib().msosm_k_fft_c2c(_ptr(inp), _ptr(out), n, batch, idist, odist, direction, _stream_handle(stream)))
+
+
+class Batch:
+    """Several engines (independent input streams) and their phase folds,
+    captured once into a hipGraph; launch() replays one full pass."""
+
+    def __init__(self, engines, inputs, num_samples, periods=None, nbins=512, stream=None):
+        n = len(engines)
+        self.engines = list(engines)
+        self.inputs = list(inputs)   # keep the device buffers alive
+        eh = (ctypes.c_void_p * n)(*[e.h.value for e in engines])
+        ih = (ctypes.c_void_p * n)(*[_ptr(x) for x in inputs])
+        ns = (ctypes.c_ulong * n)(*num_samples)
+        per = (ctypes.c_double * n)(*periods) if periods is not None else None
+        self.nbins = nbins
+        h = ctypes.c_void_p()
+        check(lib().msosm_batch_create(eh, n, ih, ns, per, nbins, _stream_handle(stream), ctypes.byref(h)))
+        self.h = h
+        k, nodes = ctypes.c_int(), ctypes.c_size_t()
+        check(lib().msosm_batch_info(self.h, ctypes.byref(k), ctypes.byref(nodes)))
+        self.graph_nodes = int(nodes.value)
+
+    def launch(self):
+        check(lib().msosm_batch_launch(self.h))
+
+    def synchronize(self):
+        check(lib().msosm_batch_synchronize(self.h))
+
+    def profiles(self, i):
+        nd = self.engines[i].plan.num_dms
+        prof = np.empty((nd, self.nbins), np.float32)
+        hits = np.empty(self.nbins, np.float32)
+        check(lib().msosm_batch_profiles(self.h, i, prof.ctypes.data, hits.ctypes.data))
+        return prof, hits
+
+    def close(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.msosm_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+def autotune_fftpoint(bw, dm, f0, min_fftpoint=1024, max_fftpoint=65536, target_samples=1 << 20, calls=3,
+                      method=METHOD_MSOSM, chirp=CHIRP_REF_F32W, form=FORM_FUSED_F16, stream=None):
+    """Measured best FFT size for these DM trials; returns (fftpoint, ns per DM-trial sample)."""
+    n = max(np.size(bw), np.size(dm), np.size(f0))
+    bwa, dma, f0a = _farr(bw, n), _farr(dm, n), _farr(f0, n)
+    fp = ctypes.POINTER(ctypes.c_float)
+    d = PlanDesc(n, bwa.ctypes.data_as(fp), dma.ctypes.data_as(fp), f0a.ctypes.data_as(fp), 0, 1, 0, method,
+                 chirp, form)
+    best, ns = ctypes.c_ulong(), ctypes.c_double()
+    check(lib().msosm_autotune_fftpoint(ctypes.byref(d), min_fftpoint, max_fftpoint, target_samples, calls,
+                                        _stream_handle(stream), ctypes.byref(best), ctypes.byref(ns)))
+    return int(best.value), float(ns.value)
+
+
+def fold_phase(a, n, period_samples, nbins, prof, hits=None, b=None, ndm=1, stride=None, n0=0, power=True,
+               stream=None):
+    """Accumulate the phase-folded profile of ndm rows of complex64 device
+    data into prof[ndm, nbins] (and sample counts into hits[nbins])."""
+    stride = n if stride is None else stride
+    check(lib().msosm_fold_phase(_ptr(a), _ptr(b), n, ndm, stride, n0, float(period_samples), nbins,
+                                 1 if power else 0, _ptr(prof), _ptr(hits), _stream_handle(stream)))
+
+
+def dm_model(bw, dm, f0, method=METHOD_MSOSM):
+    """(Nd, M_min) of one DM trial."""
+    nd, mm = ctypes.c_ulong(), ctypes.c_ulong()
+    check(lib().msosm_dm_model(bw, dm, f0, method, ctypes.byref(nd), ctypes.byref(mm)))
+    return int(nd.value), int(mm.value)
 
 
 def version():

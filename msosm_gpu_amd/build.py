@@ -24,7 +24,7 @@ LIB = os.path.join(LIBDIR, "libmsosm.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("MSOSM_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["kernels.hip", "engine.hip", "fold.hip"]
+HIP_SOURCES = ["kernels.hip", "engine.hip", "fold.hip", "batch.hip"]
 HOST_SOURCES = ["plan.cpp"]
 
 HIP_FLAGS = [
@@ -89,9 +89,19 @@ def build_library(jobs=None):
     return LIB
 
 
+def build_facade_check():
+    """C++ program written against the drop-in facade headers (tests/cpp)."""
+    src = os.path.join(ROOT, "tests", "cpp", "facade_check.cpp")
+    out = os.path.join(ROOT, "tests", "cpp", "facade_check")
+    _run(["g++", "-std=c++17", "-O2", "-Wall", "-I", INCLUDE, src, "-o", out, "-L", LIBDIR, "-lmsosm",
+          "-Wl,-rpath,$ORIGIN/../../msosm_gpu_amd/lib"])
+    return out
+
+
 def clean():
     shutil.rmtree(OBJDIR, ignore_errors=True)
 
 
 if __name__ == "__main__":
     print(build_library())
+    print(build_facade_check())

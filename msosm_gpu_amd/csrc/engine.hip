@@ -22,6 +22,7 @@
 #include <vector>
 
 #include "common.h"
+#include "engine_internal.h"
 #include "kernels.h"
 #include "msosm_c.h"
 #include "plan.h"
@@ -92,6 +93,7 @@ struct msosm_engine {
     __half2 *H16 = nullptr;         // [ndm][2M] = H/M (FUSED_F16)
     int32_t *delay32 = nullptr;     // [ndm][2M] (PLAIN)
     uint16_t *delay16 = nullptr;    // [ndm][2M] (FUSED)
+    uint32_t *delay_pk = nullptr;   // [ndm][E/2][NT] packed pairs (FUSED, one transform per workgroup)
     float2 *out = nullptr;          // [ndm][count][M] + 1 sentinel
     uint32_t *out_u16 = nullptr;    // packed output + 1 sentinel
     // PLAIN_F32
@@ -181,8 +183,8 @@ void free_engine(msosm_engine *e)
     harvest(e);
     for (auto ev : e->pool)
         hipEventDestroy(ev);
-    void *bufs[] = {e->in_stage, e->H32, e->H16, e->delay32, e->delay16, e->out, e->out_u16, e->inbuf, e->fftblk,
-                    e->ring, e->ifftblk, e->fft_scratch, e->tail, e->xring};
+    void *bufs[] = {e->in_stage, e->H32, e->H16, e->delay32, e->delay16, e->delay_pk, e->out, e->out_u16, e->inbuf,
+                    e->fftblk, e->ring, e->ifftblk, e->fft_scratch, e->tail, e->xring};
     for (void *b : bufs)
         if (b)
             hipFree(b);
@@ -336,10 +338,25 @@ int create_engine(const msosm::Plan &plan, void *stream, msosm_engine **out)
         herr = hipMemcpyAsync(tmp, p.delay.data(), sizeof(int32_t) * ndm * N, hipMemcpyHostToDevice, e->stream);
         if (herr == hipSuccess)
             herr = msosm::launch_i32_to_u16(e->delay16, tmp, (size_t)ndm * N, e->stream);
-        hipStreamSynchronize(e->stream);
-        hipFree(tmp);
+        (void)hipStreamSynchronize(e->stream);
+        (void)hipFree(tmp);
         if (herr != hipSuccess)
             return bail(set_error(MSOSM_ERR_HIP, "delay table upload: %s", hipGetErrorString(herr)));
+        if (msosm::fused_uses_packed_delay(e->log2n)) {
+            const int NT = msosm::fused_delay_threads(e->log2n);
+            const int E = (int)(N / NT);
+            std::vector<uint32_t> pk((size_t)ndm * N / 2);
+            for (int d = 0; d < ndm; d++) {
+                const int32_t *row = p.delay.data() + (size_t)d * N;
+                uint32_t *dst = pk.data() + (size_t)d * N / 2;
+                for (int j = 0; j < E / 2; j++)
+                    for (int t = 0; t < NT; t++)
+                        dst[(size_t)j * NT + t] = (uint32_t)row[t + 2 * j * NT] |
+                                                  ((uint32_t)row[t + (2 * j + 1) * NT] << 16);
+            }
+            ALLOC(e->delay_pk, pk.size());
+            HIP_TRY(hipMemcpy(e->delay_pk, pk.data(), sizeof(uint32_t) * pk.size(), hipMemcpyHostToDevice));
+        }
     }
 #undef ALLOC
     HIP_TRY(hipMemsetAsync(e->out, 0, sizeof(float2) * out_elems, e->stream));
@@ -360,7 +377,7 @@ int filter_plain(msosm_engine *e, const uint32_t *cur)
     hipStream_t s = e->stream;
     {
         StageScope st(e, MSOSM_STAGE_COPY); // overlap-save history (msosm_gpu_batch.cpp:80)
-        HIP_TRY(hipMemcpyAsync(e->inbuf, e->inbuf + (size_t)count * M, sizeof(float2) * M, hipMemcpyDeviceToDevice, s));
+        HIP_TRY(msosm::launch_copy_u32(e->inbuf, e->inbuf + (size_t)count * M, 2 * M, s));
     }
     {
         StageScope st(e, MSOSM_STAGE_UNPACK);
@@ -404,19 +421,39 @@ int filter_fused(msosm_engine *e, const uint32_t *cur)
     }
     {
         StageScope st(e, MSOSM_STAGE_COPY);
-        HIP_TRY(hipMemcpyAsync(e->tail, cur + (size_t)(count - 1) * M, sizeof(uint32_t) * M, hipMemcpyDeviceToDevice,
-                               s));
+        HIP_TRY(msosm::launch_copy_u32(e->tail, cur + (size_t)(count - 1) * M, M, s));
     }
     {
         StageScope st(e, MSOSM_STAGE_FFT_INV);
         HIP_TRY(msosm::launch_inv_fused(e->xring, f16 ? (const void *)e->H16 : (const void *)e->H32, e->delay16,
-                                        e->out, f16, e->log2n, count, ndm, e->g0, e->rx, s));
+                                        e->delay_pk, e->out, f16, e->log2n, count, ndm, e->g0, e->rx, s));
     }
     e->g0 += count;
     return MSOSM_OK;
 }
 
 } // namespace
+
+// Zero the stream history with fill kernels only (safe inside graph capture).
+int msosm::engine_reset_async(msosm_engine *e)
+{
+    const msosm::Plan &p = e->plan;
+    if (e->form == MSOSM_FORM_PLAIN_F32) {
+        HIP_TRY(msosm::launch_fill_u32(e->inbuf, 2 * (size_t)(p.count + 1) * p.M, 0, e->stream));
+        HIP_TRY(msosm::launch_fill_u32(e->ring, 2 * (size_t)p.num_dms * p.R * p.N, 0, e->stream));
+        e->current_index = 0;
+    } else {
+        size_t words = (e->form == MSOSM_FORM_FUSED_F16 ? 1 : 2) * (size_t)e->rx * p.N;
+        HIP_TRY(msosm::launch_fill_u32(e->xring, words, 0, e->stream));
+        HIP_TRY(msosm::launch_fill_u32(e->tail, p.M, 0, e->stream));
+        e->g0 = 0;
+    }
+    return MSOSM_OK;
+}
+
+hipStream_t msosm::engine_stream(msosm_engine *e) { return e->stream; }
+float2 *msosm::engine_output(msosm_engine *e) { return e->out; }
+const msosm::Plan &msosm::engine_plan(msosm_engine *e) { return e->plan; }
 
 // ===========================================================================
 // C ABI
@@ -443,6 +480,18 @@ int msosm_plan_create(const msosm_plan_desc *desc, msosm_plan **out)
 }
 
 void msosm_plan_destroy(msosm_plan *plan) { delete plan; }
+
+int msosm_dm_model(float bw, float dm, float f0, int method, unsigned long *nd, unsigned long *m_min)
+{
+    if (!(bw > 0) || !(f0 > 0) || !(dm >= 0) || (method != MSOSM_METHOD_MSOSM && method != MSOSM_METHOD_OSM))
+        return set_error(MSOSM_ERR_ARG, "dm_model: need bw > 0, f0 > 0, dm >= 0 and a known method");
+    msosm::DmModel m = msosm::make_dm_model(bw, dm, f0);
+    if (nd)
+        *nd = m.nd;
+    if (m_min)
+        *m_min = method == MSOSM_METHOD_MSOSM ? msosm::msosm_min_order(m.nd) : msosm::osm_min_order(m.nd);
+    return MSOSM_OK;
+}
 
 int msosm_plan_get_info(const msosm_plan *plan, msosm_plan_info *info)
 {
@@ -577,17 +626,9 @@ int msosm_engine_reset(msosm_engine *e)
 {
     if (!e)
         return set_error(MSOSM_ERR_ARG, "reset: NULL engine");
-    const msosm::Plan &p = e->plan;
-    if (e->form == MSOSM_FORM_PLAIN_F32) {
-        HIP_TRY(hipMemsetAsync(e->inbuf, 0, sizeof(float2) * (p.count + 1) * p.M, e->stream));
-        HIP_TRY(hipMemsetAsync(e->ring, 0, sizeof(float2) * (size_t)p.num_dms * p.R * p.N, e->stream));
-        e->current_index = 0;
-    } else {
-        size_t esz = e->form == MSOSM_FORM_FUSED_F16 ? sizeof(__half2) : sizeof(float2);
-        HIP_TRY(hipMemsetAsync(e->xring, 0, esz * (size_t)e->rx * p.N, e->stream));
-        HIP_TRY(hipMemsetAsync(e->tail, 0, sizeof(uint32_t) * p.M, e->stream));
-        e->g0 = 0;
-    }
+    int rc = msosm::engine_reset_async(e);
+    if (rc != MSOSM_OK)
+        return rc;
     HIP_TRY(hipStreamSynchronize(e->stream));
     return MSOSM_OK;
 }

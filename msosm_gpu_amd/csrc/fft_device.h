@@ -34,61 +34,68 @@ __device__ __forceinline__ float2 cmulconj(float2 a, float2 b) // a * conj(b)
 }
 __device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
 
-// cos(2 pi k / 16) for k = 0..15, folded at compile time.
-__host__ __device__ constexpr float cos16(int k)
+// cos(2 pi k / 64) at compile time (Taylor series on a reduced argument;
+// double precision, rounded to float by the caller).
+__host__ __device__ constexpr double ccos_reduced(double x)
 {
-    k &= 15;
-    return k == 0 ? 1.0f
-         : k == 1 ? 0.92387953251128674f
-         : k == 2 ? 0.70710678118654752f
-         : k == 3 ? 0.38268343236508977f
-         : k == 4 ? 0.0f
-         : k == 5 ? -0.38268343236508977f
-         : k == 6 ? -0.70710678118654752f
-         : k == 7 ? -0.92387953251128674f
-         : k == 8 ? -1.0f
-         : k == 9 ? -0.92387953251128674f
-         : k == 10 ? -0.70710678118654752f
-         : k == 11 ? -0.38268343236508977f
-         : k == 12 ? 0.0f
-         : k == 13 ? 0.38268343236508977f
-         : k == 14 ? 0.70710678118654752f
-                   : 0.92387953251128674f;
+    double term = 1.0, sum = 1.0;
+    for (int i = 1; i < 30; i++) {
+        term *= -x * x / ((2.0 * i - 1.0) * (2.0 * i));
+        sum += term;
+    }
+    return sum;
 }
-__host__ __device__ constexpr float sin16(int k) { return cos16(k + 12); }
+__host__ __device__ constexpr double cos64(int k)
+{
+    k &= 63;
+    // reduce to [0, pi/2] with exact symmetries
+    return k == 0 ? 1.0
+         : k == 16 ? 0.0
+         : k == 32 ? -1.0
+         : k == 48 ? 0.0
+         : k < 16 ? ccos_reduced(2.0 * 3.14159265358979323846 * k / 64.0)
+         : k < 32 ? -ccos_reduced(2.0 * 3.14159265358979323846 * (32 - k) / 64.0)
+         : k < 48 ? -ccos_reduced(2.0 * 3.14159265358979323846 * (k - 32) / 64.0)
+                  : ccos_reduced(2.0 * 3.14159265358979323846 * (64 - k) / 64.0);
+}
+__host__ __device__ constexpr double sin64(int k) { return cos64(k + 48); } // sin t = cos(t - pi/2)
 
-// Multiply by W_R^e (forward: exp(-2 pi i e/R), inverse: conjugate), R <= 16,
+// Multiply by W_R^e (forward: exp(-2 pi i e/R), inverse: conjugate), R <= 64,
 // with the trivial angles resolved at compile time.
 template <int R, bool INV, int E_>
 __device__ __forceinline__ float2 twiddle_const(float2 a)
 {
-    constexpr int k = ((E_ % R) * (16 / R)) & 15; // angle index in 16ths
+    constexpr int k = ((E_ % R) * (64 / R)) & 63; // angle index in 64ths
     if constexpr (k == 0) {
         return a;
-    } else if constexpr (k == 4) { // -i (fwd) / +i (inv)
+    } else if constexpr (k == 16) { // -i (fwd) / +i (inv)
         return INV ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
-    } else if constexpr (k == 8) {
+    } else if constexpr (k == 32) {
         return make_float2(-a.x, -a.y);
-    } else if constexpr (k == 12) { // +i (fwd) / -i (inv)
+    } else if constexpr (k == 48) { // +i (fwd) / -i (inv)
         return INV ? make_float2(a.y, -a.x) : make_float2(-a.y, a.x);
     } else {
-        constexpr float c = cos16(k);
-        constexpr float s = INV ? sin16(k) : -sin16(k);
+        constexpr float c = (float)cos64(k);
+        constexpr float s = INV ? (float)sin64(k) : -(float)sin64(k);
         return make_float2(a.x * c - a.y * s, a.x * s + a.y * c);
     }
 }
 
-// In-register DFT of size R (natural order in and out).
+// In-register DFT of size R, in place: natural order in, output X[k] left at
+// register pos(k) (a compile-time digit permutation, absorbed by the
+// caller's indexing, so no temporary copy of the R values is needed).
 template <int R, bool INV>
 struct Dft;
 
 template <bool INV>
 struct Dft<1, INV> {
+    __host__ __device__ static constexpr int pos(int k) { return k; }
     __device__ __forceinline__ static void run(float2 *) {}
 };
 
 template <bool INV>
 struct Dft<2, INV> {
+    __host__ __device__ static constexpr int pos(int k) { return k; }
     __device__ __forceinline__ static void run(float2 *v)
     {
         float2 a = v[0], b = v[1];
@@ -99,6 +106,7 @@ struct Dft<2, INV> {
 
 template <bool INV>
 struct Dft<4, INV> {
+    __host__ __device__ static constexpr int pos(int k) { return k; }
     __device__ __forceinline__ static void run(float2 *v)
     {
         float2 s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
@@ -112,11 +120,14 @@ struct Dft<4, INV> {
     }
 };
 
-// R = A*B with A = 4: n = B*n1 + n2, k = k1 + A*k2.
+// R = A*B with A = 4: n = B*n1 + n2, k = k1 + A*k2.  Stage 1 (DFT_A over n1
+// for every n2, then the W_R^(n2 k1) twiddle) leaves Y[n2][k1] at v[B*k1 +
+// n2]; stage 2 is an in-place DFT_B on each contiguous run v[B*k1 .. +B).
 template <int R, bool INV>
 struct Dft {
     static constexpr int A = 4;
     static constexpr int B = R / A;
+    __host__ __device__ static constexpr int pos(int k) { return B * (k % A) + Dft<B, INV>::pos(k / A); }
     template <int N2>
     __device__ __forceinline__ static void stage1(float2 *v)
     {
@@ -141,22 +152,9 @@ struct Dft {
     __device__ __forceinline__ static void run(float2 *v)
     {
         stage1_all<0>(v);
-        // now v[B*k1 + n2] holds twiddled Y[n2][k1]; DFT_B over n2 for each k1
-        float2 out[R];
 #pragma unroll
-        for (int k1 = 0; k1 < A; k1++) {
-            float2 w[B];
-#pragma unroll
-            for (int n2 = 0; n2 < B; n2++)
-                w[n2] = v[B * k1 + n2];
-            Dft<B, INV>::run(w);
-#pragma unroll
-            for (int k2 = 0; k2 < B; k2++)
-                out[k1 + A * k2] = w[k2];
-        }
-#pragma unroll
-        for (int i = 0; i < R; i++)
-            v[i] = out[i];
+        for (int k1 = 0; k1 < A; k1++)
+            Dft<B, INV>::run(v + B * k1);
     }
 };
 
@@ -234,22 +232,43 @@ __device__ __forceinline__ void fft_pass(float2 (&v)[E], float2 *lds, const floa
         const int j = t + q * NT;
         const int k = j & (NS - 1);
         if constexpr (P > 0) {
+            // W^r for r = c + 4b + 16a from three lookups (W, W^4, W^16) and
+            // products of base-4 digits: at most 4 roundings per twiddle.
+            const int e1 = k << (LOG2N - LOGNS - LOGR);
+            const float2 w1 = twiddle_lookup<LOG2N, E, INV>(twlo, twhi, e1);
+            const float2 w2 = cmul(w1, w1);
+            const float2 w3 = cmul(w2, w1);
+            float2 w4 = make_float2(1.f, 0.f), w8 = w4, w12 = w4, w16 = w4;
+            if constexpr (R > 4) {
+                w4 = twiddle_lookup<LOG2N, E, INV>(twlo, twhi, (4 * e1) & (S::N - 1));
+                w8 = cmul(w4, w4);
+                w12 = cmul(w8, w4);
+            }
+            if constexpr (R > 16)
+                w16 = twiddle_lookup<LOG2N, E, INV>(twlo, twhi, (16 * e1) & (S::N - 1));
 #pragma unroll
             for (int r = 1; r < R; r++) {
-                int e = (r * k) << (LOG2N - LOGNS - LOGR);
-                u[r] = cmul(u[r], twiddle_lookup<LOG2N, E, INV>(twlo, twhi, e));
+                const int c = r & 3, b = (r >> 2) & 3, a = r >> 4;
+                float2 w = c == 1 ? w1 : c == 2 ? w2 : w3;
+                if (c == 0)
+                    w = b == 1 ? w4 : b == 2 ? w8 : w12;
+                else if (b != 0)
+                    w = cmul(w, b == 1 ? w4 : b == 2 ? w8 : w12);
+                if (a != 0)
+                    w = (c == 0 && b == 0) ? w16 : cmul(w, w16);
+                u[r] = cmul(u[r], w);
             }
         }
         Dft<R, INV>::run(u);
         if constexpr (P == NPASS - 1) {
 #pragma unroll
             for (int r = 0; r < R; r++)
-                v[q + r * Q] = u[r];
+                v[q + r * Q] = u[Dft<R, INV>::pos(r)];
         } else {
             const int base = ((j >> LOGNS) << (LOGNS + LOGR)) + k;
 #pragma unroll
             for (int r = 0; r < R; r++)
-                lds[lidx<LOGE>(base + r * NS)] = u[r];
+                lds[lidx<LOGE>(base + r * NS)] = u[Dft<R, INV>::pos(r)];
         }
     }
     if constexpr (P < NPASS - 1) {
@@ -267,14 +286,23 @@ __device__ __forceinline__ void fft_block(float2 (&v)[E], float2 *lds, const flo
     using S = FftShape<LOG2N, E>;
     if constexpr (S::NPASS == 1) {
         Dft<S::N, INV>::run(v);
+        float2 o[E];
+#pragma unroll
+        for (int k = 0; k < E; k++)
+            o[k] = v[Dft<S::N, INV>::pos(k)];
+#pragma unroll
+        for (int k = 0; k < E; k++)
+            v[k] = o[k];
     } else {
         __syncthreads(); // LDS may still be read by a previous transform
         fft_pass<LOG2N, E, INV, 0>(v, lds, twlo, twhi, t);
     }
 }
 
-// Elements per thread for a transform size.
-__host__ __device__ constexpr int fft_e(int log2n) { return log2n >= 4 ? 16 : (1 << log2n); }
+// Elements per thread for a transform size: 16 (radix-16 passes), 32 for
+// 2^14 points (512 threads, radix 32-32-16: one LDS exchange fewer and a
+// 256-register budget per thread).
+__host__ __device__ constexpr int fft_e(int log2n) { return log2n >= 14 ? 32 : log2n >= 4 ? 16 : (1 << log2n); }
 // Transforms per workgroup (fill ~256 threads for small sizes).
 __host__ __device__ constexpr int fft_tpb(int log2n)
 {

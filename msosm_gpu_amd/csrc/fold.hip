@@ -67,7 +67,51 @@ __global__ void k_fold_bins(float *__restrict__ folded, const float *__restrict_
         float sum = 0.0f;
         for (int j = 0; j < bpt; j++)
             sum = sum + raw[i * (unsigned long)bpt + j];
-        folded[i] = __fdiv_rn(sum, (float)bpt);
+        folded[i] = sum / (float)bpt; // IEEE division
+    }
+}
+
+// Phase fold: sample i of DM trial dm (global index n0 + i) lands in bin
+// floor(frac((n0 + i) / P) * nbins), the phase evaluated in FP64.  Each
+// workgroup builds a histogram in LDS (float atomics) and adds it to the
+// global profile once.  value = sqrt(|a|^2+|b|^2) (the reference's
+// intensity, gpu_kernel.cu:352) or |a|^2+|b|^2 (power).
+__global__ void k_fold_phase(const float2 *__restrict__ a, const float2 *__restrict__ b, long long n, long long stride,
+                             long long n0, double inv_period, int nbins, int power, float *__restrict__ prof,
+                             float *__restrict__ hits)
+{
+    extern __shared__ float hist[]; // nbins values (+ nbins hit counts for trial 0)
+    const int dm = blockIdx.y;
+    const bool count_hits = (hits != nullptr) && dm == 0;
+    const int nh = count_hits ? 2 * nbins : nbins;
+    for (int i = threadIdx.x; i < nh; i += blockDim.x)
+        hist[i] = 0.0f;
+    __syncthreads();
+    const float2 *ad = a + (long long)dm * stride;
+    const float2 *bd = b ? b + (long long)dm * stride : nullptr;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        float2 v = ad[i];
+        float p = v.x * v.x + v.y * v.y;
+        if (bd) {
+            float2 w = bd[i];
+            p += w.x * w.x + w.y * w.y;
+        }
+        if (!power)
+            p = sqrtf(p);
+        double ph = (double)(n0 + i) * inv_period;
+        ph -= floor(ph);
+        int bin = (int)(ph * (double)nbins);
+        bin = bin < nbins ? bin : nbins - 1;
+        atomicAdd(&hist[bin], p);
+        if (count_hits)
+            atomicAdd(&hist[nbins + bin], 1.0f);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < nbins; i += blockDim.x) {
+        atomicAdd(&prof[(long long)dm * nbins + i], hist[i]);
+        if (count_hits)
+            atomicAdd(&hits[i], hist[nbins + i]);
     }
 }
 
@@ -413,6 +457,29 @@ int msosm_fold_info(msosm_fold *f, unsigned long *period_samples, unsigned long 
         *time_bin = f->time_bin;
     if (ready)
         *ready = f->ready ? 1 : 0;
+    return MSOSM_OK;
+}
+
+int msosm_fold_phase(const void *a, const void *b, size_t n, int ndm, size_t stride, long long n0,
+                     double period_samples, int nbins, int power, float *prof, float *hits, void *stream)
+{
+    if (!a || !prof || n == 0 || ndm < 1 || nbins < 1 || nbins > 8192 || !(period_samples > 0) ||
+        (ndm > 1 && stride < n))
+        return ferr(MSOSM_ERR_ARG, "fold_phase: bad argument (need n > 0, 1 <= nbins <= 8192, period > 0, "
+                                   "stride >= n)");
+    if (ndm > 65535)
+        return ferr(MSOSM_ERR_ARG, "fold_phase: at most 65535 DM trials per launch");
+    const int threads = 256;
+    long long gx = (long long)((n + threads * 16 - 1) / (threads * 16));
+    if (gx > 1024)
+        gx = 1024;
+    if (gx < 1)
+        gx = 1;
+    size_t lds = sizeof(float) * (size_t)nbins * (hits ? 2 : 1);
+    hipLaunchKernelGGL(k_fold_phase, dim3((unsigned)gx, ndm), dim3(threads), lds, (hipStream_t)stream,
+                       (const float2 *)a, (const float2 *)b, (long long)n, (long long)stride, n0,
+                       1.0 / period_samples, nbins, power, prof, hits);
+    FHIP(hipGetLastError());
     return MSOSM_OK;
 }
 

@@ -23,8 +23,13 @@ hipError_t launch_fwd_u16(const uint32_t *cur, const uint32_t *tail, void *ring,
                           long long g0, int rx, hipStream_t s);
 // Fused inverse: per (DM, block) gather X_{g-d(n)}[n] * H[n] from the ring,
 // inverse FFT, keep the last M samples, scale; out layout [dm][count][M].
-hipError_t launch_inv_fused(const void *ring, const void *H, const uint16_t *delay, float2 *out, bool f16,
-                            int log2n, int count, int ndm, long long g0, int rx, hipStream_t s);
+// dpk: for sizes with one transform per workgroup (fused_uses_packed_delay),
+// the delays in thread-major packed layout dpk[dm][j][t] = d(t + 2j*NT) |
+// d(t + (2j+1)*NT) << 16 with NT = fused_delay_threads(log2n); else unused.
+hipError_t launch_inv_fused(const void *ring, const void *H, const uint16_t *delay, const uint32_t *dpk, float2 *out,
+                            bool f16, int log2n, int count, int ndm, long long g0, int rx, hipStream_t s);
+int fused_delay_threads(int log2n);
+bool fused_uses_packed_delay(int log2n);
 bool fused_supported(int log2n);
 
 // Element kernels (reference gpu_kernel.cu semantics).
@@ -47,5 +52,12 @@ hipError_t launch_zero_pad(float2 *dst, const float2 *src, unsigned long M, int 
 // dst = half2(src * scale)
 hipError_t launch_to_half(__half2 *dst, const float2 *src, size_t n, float scale, hipStream_t s);
 hipError_t launch_i32_to_u16(uint16_t *dst, const int32_t *src, size_t n, hipStream_t s);
+// Capturable replacements for hipMemsetAsync / device-to-device copies
+// (graph replays stay correct with kernel nodes).
+hipError_t launch_fill_u32(void *dst, size_t words, uint32_t value, hipStream_t s);
+hipError_t launch_copy_u32(void *dst, const void *src, size_t words, hipStream_t s);
+// Deterministic pseudo-random packed uint16 samples (offset binary around
+// 32768, never 0) for timing runs.
+hipError_t launch_fill_random_u16(uint32_t *dst, size_t n, uint32_t seed, hipStream_t s);
 
 } // namespace msosm

@@ -12,6 +12,8 @@
 
 #include <hip/hip_fp16.h>
 
+#include <cstdlib>
+
 #include "fft_device.h"
 
 namespace msosm {
@@ -85,7 +87,7 @@ __global__ void k_intensity(const float2 *__restrict__ a, const float2 *__restri
             s = s + q.x * q.x;
             s = s + q.y * q.y;
         }
-        out[i] = __fsqrt_rn(s);
+        out[i] = sqrtf(s); // correctly rounded (v_sqrt_f32 alone is not)
     }
 }
 
@@ -200,6 +202,32 @@ __global__ void k_i32_to_u16(uint16_t *__restrict__ dst, const int32_t *__restri
 {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
         dst[i] = (uint16_t)src[i];
+}
+
+__global__ void k_fill_u32(uint32_t *__restrict__ dst, size_t n, uint32_t v)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = v;
+}
+
+__global__ void k_copy_u32(uint32_t *__restrict__ dst, const uint32_t *__restrict__ src, size_t n)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+__global__ void k_fill_random_u16(uint32_t *__restrict__ dst, size_t n, uint32_t seed)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        uint32_t h = (uint32_t)i * 2654435761u ^ seed;
+        h ^= h >> 16;
+        h *= 0x7feb352du;
+        h ^= h >> 15;
+        h *= 0x846ca68bu;
+        h ^= h >> 16;
+        uint32_t a = 32768u - 2048u + (h & 4095u), b = 32768u - 2048u + ((h >> 12) & 4095u);
+        dst[i] = a | (b << 16);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -395,6 +423,7 @@ __global__ __launch_bounds__(fft_threads(LOG2N)) void k_fwd_u16(
 // Y[n] = X_{g-d(n)}[n] * H[n] (zero before the stream start), inverse FFT,
 // keep points [M, 2M), scale: FP32 ring/H -> 1/N^2 (the reference's two 1/N
 // factors, gpu_kernel.cu:169,251), FP16 ring (X/N) and H (H/M) -> 1/2.
+// This variant serves the small sizes (several transforms per workgroup).
 template <int LOG2N, bool F16>
 __global__ __launch_bounds__(fft_threads(LOG2N)) void k_inv_fused(
     const void *__restrict__ ring, const void *__restrict__ H, const uint16_t *__restrict__ delay,
@@ -409,31 +438,31 @@ __global__ __launch_bounds__(fft_threads(LOG2N)) void k_inv_fused(
     const int tl = threadIdx.x / S::NT, t = threadIdx.x % S::NT;
     if constexpr (S::NPASS > 1)
         fill_twiddles<LOG2N, E>(twlo, twhi, threadIdx.x, blockDim.x);
-    const long long u = (long long)blockIdx.x * TPB + tl;
-    const long long units = (long long)ndm * count;
-    const bool valid = u < units;
-    const int dm = valid ? (int)(u / count) : 0;
-    const int b = valid ? (int)(u % count) : 0;
+    const int u = blockIdx.x * TPB + tl;
+    const bool valid = u < ndm * count;
+    const int dm = valid ? u / count : 0;
+    const int b = valid ? u - dm * count : 0;
     const long long g = g0 + b;
+    const int gslot = (int)(g % rx);
     const uint16_t *drow = delay + (long long)dm * N;
     float2 v[E];
 #pragma unroll
     for (int m = 0; m < E; m++) {
         const int n = t + m * S::NT;
         float2 x = make_float2(0.f, 0.f);
-        if (valid) {
-            long long src = g - (long long)drow[n];
-            if (src >= 0) {
-                long long off = (src % rx) * N + n;
-                if constexpr (F16) {
-                    float2 xs = __half22float2(reinterpret_cast<const __half2 *>(ring)[off]);
-                    float2 h = __half22float2(reinterpret_cast<const __half2 *>(H)[(long long)dm * N + n]);
-                    x = cmul(xs, h);
-                } else {
-                    float2 xs = reinterpret_cast<const float2 *>(ring)[off];
-                    float2 h = reinterpret_cast<const float2 *>(H)[(long long)dm * N + n];
-                    x = cmul(xs, h);
-                }
+        const int d = drow[n];
+        if (valid && g >= d) {
+            int sl = gslot - d;
+            sl += sl < 0 ? rx : 0;
+            const long long off = (long long)sl * N + n;
+            if constexpr (F16) {
+                float2 xs = __half22float2(reinterpret_cast<const __half2 *>(ring)[off]);
+                float2 h = __half22float2(reinterpret_cast<const __half2 *>(H)[(long long)dm * N + n]);
+                x = cmul(xs, h);
+            } else {
+                float2 xs = reinterpret_cast<const float2 *>(ring)[off];
+                float2 h = reinterpret_cast<const float2 *>(H)[(long long)dm * N + n];
+                x = cmul(xs, h);
             }
         }
         v[m] = x;
@@ -445,6 +474,107 @@ __global__ __launch_bounds__(fft_threads(LOG2N)) void k_inv_fused(
 #pragma unroll
         for (int m = E / 2; m < E; m++)
             dst[t + m * S::NT - M] = cscale(v[m], scale);
+    }
+}
+
+template <bool F16>
+struct RingElem;
+template <>
+struct RingElem<true> {
+    using T = __half2;
+    __device__ static float2 f2(T x) { return __half22float2(x); }
+    __device__ static T zero() { return __floats2half2_rn(0.f, 0.f); }
+};
+template <>
+struct RingElem<false> {
+    using T = float2;
+    __device__ static float2 f2(T x) { return x; }
+    __device__ static T zero() { return make_float2(0.f, 0.f); }
+};
+
+// Fused inverse for one transform per workgroup (2^12..2^14 points): the
+// workgroup runs NB consecutive blocks of one DM trial and issues the gather
+// of block i+1 before the transform of block i, so the ring loads overlap the
+// LDS passes.  Only the prefetched ring values stay in registers across the
+// transform; H and the delays are re-read per block (L2 hits).  The delays
+// come in a thread-major packed layout dpk[dm][j][t] = d(t + 2j NT) |
+// d(t + (2j+1) NT) << 16 so each thread reads E/2 coalesced words.  Work
+// items are ordered so that each XCD (workgroups are dealt to the 8 XCDs
+// round robin) gets a contiguous range of DM trials and keeps their tables in
+// its L2.
+template <int LOG2N, bool F16, int NB, bool PREFETCH>
+__global__ __launch_bounds__(fft_threads(LOG2N)) void k_inv_fused_nb(
+    const void *__restrict__ ring, const void *__restrict__ H, const uint32_t *__restrict__ dpk,
+    float2 *__restrict__ out, int count, int ndm, long long g0, int rx)
+{
+    constexpr int E = fft_e(LOG2N);
+    using S = FftShape<LOG2N, E>;
+    static_assert(fft_tpb(LOG2N) == 1, "one transform per workgroup");
+    constexpr long long N = S::N, M = S::N / 2;
+    constexpr int NT = S::NT;
+    using R = RingElem<F16>;
+    using T = typename R::T;
+    __shared__ float2 smem[S::NPAD + S::TWLO + S::TWHI];
+    float2 *twlo = smem + S::NPAD, *twhi = twlo + S::TWLO;
+    fill_twiddles<LOG2N, E>(twlo, twhi, threadIdx.x, blockDim.x);
+    const int t = threadIdx.x;
+    const int chunks = (count + NB - 1) / NB;
+    const int G = gridDim.x;
+    int w = blockIdx.x;
+    if ((G & 7) == 0)
+        w = (w & 7) * (G >> 3) + (w >> 3);
+    if (w >= ndm * chunks)
+        return; // the whole workgroup leaves together (w is uniform)
+    const int dm = w / chunks;
+    const int b0 = (w - dm * chunks) * NB;
+    const int nb = min(NB, count - b0);
+    const T *ringT = reinterpret_cast<const T *>(ring);
+    const T *hrow = reinterpret_cast<const T *>(H) + (long long)dm * N;
+    const uint32_t *drow = dpk + (long long)dm * N / 2;
+    long long g = g0 + b0;
+    int slot = (int)(g % rx);
+    T raw[E];
+    // ring offsets fit 32 bits (rx * N elements): one VGPR per address
+    auto gather = [&]() {
+#pragma unroll
+        for (int j = 0; j < E / 2; j++) {
+            const uint32_t pair = drow[j * NT + t];
+#pragma unroll
+            for (int q = 0; q < 2; q++) {
+                const int m = 2 * j + q;
+                const int d = (int)((pair >> (16 * q)) & 0xFFFFu);
+                int sl = slot - d;
+                sl += sl < 0 ? rx : 0;
+                const uint32_t off = (uint32_t)sl * (uint32_t)N + (uint32_t)(t + m * NT);
+                raw[m] = g >= d ? ringT[off] : R::zero();
+            }
+        }
+    };
+    if constexpr (PREFETCH)
+        gather();
+    const float scale = F16 ? 0.5f : 1.0f / ((float)N * (float)N);
+    for (int i = 0; i < nb; i++) {
+        if constexpr (!PREFETCH)
+            gather();
+        float2 v[E];
+#pragma unroll
+        for (int m = 0; m < E; m++)
+            v[m] = cmul(R::f2(raw[m]), R::f2(hrow[t + m * NT]));
+        if constexpr (PREFETCH) {
+            if (i + 1 < nb) {
+                g += 1;
+                slot = slot + 1 == rx ? 0 : slot + 1;
+                gather();
+            }
+        } else {
+            g += 1;
+            slot = slot + 1 == rx ? 0 : slot + 1;
+        }
+        fft_block<LOG2N, E, true>(v, smem, twlo, twhi, t);
+        float2 *dst = out + ((long long)dm * count + b0 + i) * M;
+#pragma unroll
+        for (int m = E / 2; m < E; m++)
+            dst[t + m * NT - M] = cscale(v[m], scale);
     }
 }
 
@@ -604,20 +734,64 @@ static hipError_t fwd_u16(const uint32_t *cur, const uint32_t *tail, void *ring,
     return hipSuccess;
 }
 
+constexpr int kInvBlocksPerWg = 8;
+
+// Inverse-kernel variant for the sizes with one transform per workgroup:
+// 0 = one (DM, block) per workgroup, 1 = kInvBlocksPerWg blocks per
+// workgroup, 2 = the same with the next block's gather prefetched (default).
+// MSOSM_INV_KERNEL overrides the choice (tuning runs).
+static int inv_variant()
+{
+    static int v = [] {
+        const char *e = getenv("MSOSM_INV_KERNEL");
+        int x = e ? atoi(e) : 2;
+        return (x >= 0 && x <= 2) ? x : 2;
+    }();
+    return v;
+}
+
+template <int LOG2N, bool F16, bool PF>
+static void launch_nb(const void *ring, const void *H, const uint32_t *dpk, float2 *out, int count, int ndm,
+                      long long g0, int rx, hipStream_t s)
+{
+    constexpr int threads = fft_threads(LOG2N);
+    const long long chunks = (count + kInvBlocksPerWg - 1) / kInvBlocksPerWg;
+    const long long grid = chunks * ndm;
+    hipLaunchKernelGGL((k_inv_fused_nb<LOG2N, F16, kInvBlocksPerWg, PF>), dim3((unsigned)grid), dim3(threads), 0, s,
+                       ring, H, dpk, out, count, ndm, g0, rx);
+}
+
 template <int LOG2N>
-static hipError_t inv_fused(const void *ring, const void *H, const uint16_t *delay, float2 *out, bool f16,
-                            int count, int ndm, long long g0, int rx, hipStream_t s)
+static hipError_t inv_fused(const void *ring, const void *H, const uint16_t *delay, const uint32_t *dpk, float2 *out,
+                            bool f16, int count, int ndm, long long g0, int rx, hipStream_t s)
 {
     constexpr int E = fft_e(LOG2N);
     constexpr int threads = FftShape<LOG2N, E>::NT * fft_tpb(LOG2N);
-    const long long units = (long long)count * ndm;
-    const long long grid = (units + fft_tpb(LOG2N) - 1) / fft_tpb(LOG2N);
-    if (f16)
-        hipLaunchKernelGGL((k_inv_fused<LOG2N, true>), dim3((unsigned)grid), dim3(threads), 0, s, ring, H, delay,
-                           out, count, ndm, g0, rx);
-    else
-        hipLaunchKernelGGL((k_inv_fused<LOG2N, false>), dim3((unsigned)grid), dim3(threads), 0, s, ring, H, delay,
-                           out, count, ndm, g0, rx);
+    const int var = inv_variant();
+    if (fft_tpb(LOG2N) == 1 && var > 0) {
+        if constexpr (fft_tpb(LOG2N) == 1) {
+            if (f16) {
+                if (var == 2)
+                    launch_nb<LOG2N, true, true>(ring, H, dpk, out, count, ndm, g0, rx, s);
+                else
+                    launch_nb<LOG2N, true, false>(ring, H, dpk, out, count, ndm, g0, rx, s);
+            } else {
+                if (var == 2)
+                    launch_nb<LOG2N, false, true>(ring, H, dpk, out, count, ndm, g0, rx, s);
+                else
+                    launch_nb<LOG2N, false, false>(ring, H, dpk, out, count, ndm, g0, rx, s);
+            }
+        }
+    } else {
+        const long long units = (long long)count * ndm;
+        const long long grid = (units + fft_tpb(LOG2N) - 1) / fft_tpb(LOG2N);
+        if (f16)
+            hipLaunchKernelGGL((k_inv_fused<LOG2N, true>), dim3((unsigned)grid), dim3(threads), 0, s, ring, H, delay,
+                               out, count, ndm, g0, rx);
+        else
+            hipLaunchKernelGGL((k_inv_fused<LOG2N, false>), dim3((unsigned)grid), dim3(threads), 0, s, ring, H,
+                               delay, out, count, ndm, g0, rx);
+    }
     MSOSM_LAUNCH_CHECK();
     return hipSuccess;
 }
@@ -648,10 +822,13 @@ hipError_t launch_fwd_u16(const uint32_t *cur, const uint32_t *tail, void *ring,
 #undef CALL_F
 }
 
-hipError_t launch_inv_fused(const void *ring, const void *H, const uint16_t *delay, float2 *out, bool f16,
-                            int log2n, int count, int ndm, long long g0, int rx, hipStream_t s)
+int fused_delay_threads(int log2n) { return (1 << log2n) / fft_e(log2n); }
+bool fused_uses_packed_delay(int log2n) { return log2n >= 2 && log2n <= kMaxLog2SinglePass && fft_tpb(log2n) == 1; }
+
+hipError_t launch_inv_fused(const void *ring, const void *H, const uint16_t *delay, const uint32_t *dpk, float2 *out,
+                            bool f16, int log2n, int count, int ndm, long long g0, int rx, hipStream_t s)
 {
-#define CALL_I(L) inv_fused<L>(ring, H, delay, out, f16, count, ndm, g0, rx, s)
+#define CALL_I(L) inv_fused<L>(ring, H, delay, dpk, out, f16, count, ndm, g0, rx, s)
     MSOSM_DISPATCH_2_14(log2n, CALL_I)
 #undef CALL_I
 }
@@ -721,6 +898,25 @@ hipError_t launch_zero_pad(float2 *dst, const float2 *src, unsigned long M, int 
 hipError_t launch_to_half(__half2 *dst, const float2 *src, size_t n, float scale, hipStream_t s)
 {
     hipLaunchKernelGGL(k_to_half, dim3(grid_for(n)), dim3(kBlock), 0, s, dst, src, n, scale);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_u32(void *dst, size_t words, uint32_t value, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(words, 4)), dim3(kBlock), 0, s, (uint32_t *)dst, words, value);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy_u32(void *dst, const void *src, size_t words, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_copy_u32, dim3(grid_for(words)), dim3(kBlock), 0, s, (uint32_t *)dst,
+                       (const uint32_t *)src, words);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_random_u16(uint32_t *dst, size_t n, uint32_t seed, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fill_random_u16, dim3(grid_for(n, 4)), dim3(kBlock), 0, s, dst, n, seed);
     return hipGetLastError();
 }
 

@@ -1,0 +1,127 @@
+// facade_check.cpp -- drives the C++ drop-in facade the way the reference's
+// drivers drive its classes (test_processing_speed_msosm.cpp,
+// multiple_dm_concurrent_msosm.cpp, msosm_psrdada.cpp) and writes every
+// output to files that tests/test_gpu_facade.py checks against the oracle.
+//
+// usage: facade_check <input.u16> <num_samples> <out_prefix>
+#include <cstdio>
+#include <fstream>
+#include <vector>
+
+#include "msosm/fold_gpu.h"
+#include "msosm/gpu_kernel.h"
+#include "msosm/msosm_gpu_batch.h"
+#include "msosm/msosm_gpu_concurrent.h"
+#include "msosm/osm_gpu.h"
+
+template <class T>
+static void dump(const std::string &path, const T *p, size_t n)
+{
+    std::ofstream f(path, std::ios::binary | std::ios::app);
+    f.write(reinterpret_cast<const char *>(p), sizeof(T) * n);
+}
+
+int main(int argc, char **argv)
+{
+    if (argc < 4) {
+        std::fprintf(stderr, "usage: %s input.u16 num_samples out_prefix\n", argv[0]);
+        return 2;
+    }
+    const std::string pre = argv[3];
+    const unsigned long L = std::stoul(argv[2]);
+    std::vector<uint16_pair> in(L);
+    {
+        std::ifstream f(argv[1], std::ios::binary);
+        f.read(reinterpret_cast<char *>(in.data()), sizeof(uint16_pair) * L);
+        if (!f) {
+            std::fprintf(stderr, "cannot read %lu samples from %s\n", L, argv[1]);
+            return 2;
+        }
+    }
+    for (const char *s : {".batch.c64", ".conc.c64", ".conc.u16", ".osm.c64", ".loop.c64", ".fold.f32", ".inten.f32"})
+        std::remove((pre + s).c_str());
+    std::ofstream meta(pre + ".meta");
+
+    // single DM (msosm_gpu_batch.h), 16 MHz, DM 10, f0 1392 MHz, N 1024
+    const float bw = 16e6f, f0 = 1392e6f;
+    MSOSM_GPU_BATCH one(bw, 10.0f, f0);
+    one.initialize_uint16(1024, 8);
+    meta << "batch M " << one.M << " M_min " << one.M_min << " Nd " << one.Nd << " K " << one.delaycount
+         << " count " << one.count << "\n";
+    std::vector<Complex> out((size_t)one.count * one.M);
+    const unsigned long per = (unsigned long)one.count * one.M;
+    for (unsigned long c = 0; c + per <= L; c += per) {
+        one.filter_block_uint16(in.data() + c);
+        one.get_output(out.data());
+        dump(pre + ".batch.c64", out.data(), out.size());
+    }
+
+    // three DM trials in one engine (msosm_gpu_concurrent.h), sentinel layout
+    float bws[3] = {bw, bw, bw}, dms[3] = {5.0f, 10.0f, 20.0f}, f0s[3] = {f0, f0, f0};
+    MSOSM_GPU_DM_concurrent conc(bws, dms, f0s, 3);
+    conc.initialize_uint16(1024, 8, 0, false);
+    meta << "conc M " << conc.M_common << " count " << conc.count << " K " << conc.delaycount[0] << " "
+         << conc.delaycount[1] << " " << conc.delaycount[2] << "\n";
+    const size_t tot = (size_t)conc.count * conc.M_common * 3;
+    std::vector<Complex> cbuf(tot + 1);
+    std::vector<uint16_pair> ubuf(tot + 1);
+    for (unsigned long c = 0; c + per <= L; c += per) {
+        conc.filter_block_uint16(in.data() + c);
+        cbuf[tot] = Complex{1.0f, 1.0f};
+        conc.get_output(cbuf.data());
+        conc.wait_for_cpu = false; // what the consumer thread does once it has the block
+        if (cbuf[tot].x != 0.0f)
+            return 3; // sentinel must read 0 once the data is there
+        dump(pre + ".conc.c64", cbuf.data(), tot);
+        conc.get_output(ubuf.data());
+        conc.wait_for_cpu = false;
+        dump(pre + ".conc.u16", ubuf.data(), tot);
+    }
+
+    // DM loop engine: trial i handed out separately
+    MSOSM_GPU_DM_BATCH_loop loop(bws, dms, f0s, 3);
+    loop.initialize_uint16(1024, 8);
+    std::vector<std::vector<Complex>> lo(3, std::vector<Complex>(per));
+    Complex *lp[3] = {lo[0].data(), lo[1].data(), lo[2].data()};
+    for (unsigned long c = 0; c + per <= L; c += per) {
+        loop.input_process(in.data() + c);
+        for (int i = 0; i < 3; i++)
+            loop.filter_block_uint16(lp, i);
+        for (int i = 0; i < 3; i++)
+            dump(pre + ".loop.c64", lp[i], per);
+    }
+
+    // single-segment OSM at DM 2 (M = 2 * nextpow2(Nd))
+    OSM_GPU_BATCH osm(bw, 2.0f, f0);
+    osm.initialize_uint16(0, 2);
+    meta << "osm M " << osm.M << " count " << osm.count << " Nd " << osm.Nd << "\n";
+    const unsigned long operr = (unsigned long)osm.count * osm.M;
+    std::vector<Complex> oout(operr);
+    for (unsigned long c = 0; c + operr <= L; c += operr) {
+        osm.filter_block_uint16(in.data() + c);
+        osm.get_output(oout.data());
+        dump(pre + ".osm.c64", oout.data(), oout.size());
+    }
+
+    // fold of two "polarisations" (msosm_psrdada.cpp:105-142): same input
+    MSOSM_GPU_BATCH p1(bw, 10.0f, f0), p2(bw, 10.0f, f0);
+    p1.initialize_uint16(1024, 8);
+    p2.initialize_uint16(1024, 8);
+    Fold_GPU fold(0.0005f, bw, per, (pre + ".fold.txt"), 100);
+    fold.discard_samples(p1.Nd);
+    for (unsigned long c = 0; c + per <= L; c += per) {
+        p1.filter_block_uint16(in.data() + c);
+        p2.filter_block_uint16(in.data() + c);
+        p1.synchronize();
+        p2.synchronize();
+        fold.calculate_intensity<MSOSM_GPU_BATCH>(&p1, &p2);
+        dump(pre + ".inten.f32", fold.total_intensity, per);
+        fold.fold_data();
+    }
+    fold.fold_data_bins();
+    fold.write_to_file();
+    dump(pre + ".fold.f32", fold.folded_data, fold.time_bin);
+    meta << "fold P " << fold.period_samples << " time_bin " << fold.time_bin << "\n";
+    std::printf("facade_check ok\n");
+    return 0;
+}

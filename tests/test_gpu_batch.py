@@ -1,0 +1,88 @@
+"""Uneven multi-stream batch in one hipGraph (engines with different
+bandwidths, DM trials, FFT sizes and lengths, each call phase-folded) against
+the same engines run eagerly, and the FFT-size autotuner."""
+import numpy as np
+import pytest
+
+import msosm_gpu_amd as ms
+
+pytestmark = pytest.mark.gpu
+
+STREAMS = [
+    # bw, f0, dms, fftpoint, count, calls, form, period (samples)
+    (8e6, 1396e6, [30.0], 1024, 8, 6, ms.FORM_FUSED_F16, 1777.7),
+    (16e6, 1392e6, [10.0, 25.0, 60.0], 2048, 4, 5, ms.FORM_FUSED_F32, 4321.0),
+    (32e6, 1384e6, [5.0, 150.0], 4096, 3, 4, ms.FORM_PLAIN_F32, 10000.5),
+    (64e6, 1368e6, list(np.linspace(0, 40, 7)), 16384, 2, 3, ms.FORM_FUSED_F16, 64000.25),
+]
+
+
+def _inputs(device):
+    import torch
+    out = []
+    for k, (bw, f0, dms, fp, count, calls, form, per) in enumerate(STREAMS):
+        n = count * (fp // 2) * calls
+        rng = np.random.Generator(np.random.Philox(100 + k))
+        u = np.clip(np.round(32768 + 800 * rng.standard_normal(2 * n)), 1, 65535).astype(np.uint16)
+        out.append((u, torch.from_numpy(u.view(np.int32)).to(device)))
+    return out
+
+
+def _eager(device, k, dev_in, nbins):
+    import torch
+    bw, f0, dms, fp, count, calls, form, per = STREAMS[k]
+    p = ms.Plan(bw, dms, f0, fftpoint=fp, count=count, form=form)
+    e = ms.Engine(p)
+    n = count * p.M
+    prof = torch.zeros((len(dms), nbins), dtype=torch.float32, device=device)
+    hits = torch.zeros(nbins, dtype=torch.float32, device=device)
+    for c in range(calls):
+        e.filter_u16(dev_in[c * n:(c + 1) * n])
+        e.synchronize()
+        ms.fold_phase(e.output_ptr(), n, per, nbins, prof, hits, ndm=len(dms), stride=n, n0=c * n)
+        torch.cuda.synchronize()
+    last = e.output_c64()
+    e.close()
+    return prof.cpu().numpy(), hits.cpu().numpy(), last
+
+
+def test_batch_graph_matches_eager(device):
+    import torch
+    nbins = 64
+    ins = _inputs(device)
+    engines, samples, periods = [], [], []
+    for k, (bw, f0, dms, fp, count, calls, form, per) in enumerate(STREAMS):
+        engines.append(ms.Engine(ms.Plan(bw, dms, f0, fftpoint=fp, count=count, form=form)))
+        samples.append(count * (fp // 2) * calls)
+        periods.append(per)
+    batch = ms.Batch(engines, [d for _, d in ins], samples, periods, nbins=nbins)
+    assert batch.graph_nodes > sum(s[5] for s in STREAMS)
+    ref = [_eager(device, k, ins[k][1], nbins) for k in range(len(STREAMS))]
+    for replay in range(3):
+        batch.launch()
+        batch.synchronize()
+        for k in range(len(STREAMS)):
+            prof, hits = batch.profiles(k)
+            rprof, rhits, rlast = ref[k]
+            np.testing.assert_array_equal(hits, rhits)
+            assert np.max(np.abs(prof - rprof) / np.maximum(np.abs(rprof), 1e-30)) < 1e-5, (replay, k)
+            np.testing.assert_array_equal(engines[k].output_c64(), rlast)
+        # eager work between replays
+        x = torch.randn(1 << 20, device=device)
+        torch.cuda.synchronize()
+        assert float((x * 0).sum()) == 0.0
+    batch.close()
+    for e in engines:
+        e.close()
+
+
+def test_autotune_picks_a_valid_size(device):
+    bw, f0, dms = 64e6, 1368e6, [20.0, 80.0, 300.0]
+    best, ns = ms.autotune_fftpoint(bw, dms, f0, min_fftpoint=1024, max_fftpoint=16384, target_samples=1 << 19,
+                                    calls=2)
+    mmin = max(ms.dm_model(bw, d, f0)[1] for d in dms)
+    assert best & (best - 1) == 0 and 2 * mmin <= best <= 16384
+    assert ns > 0
+    again = ms.autotune_fftpoint(bw, dms, f0, min_fftpoint=1024, max_fftpoint=16384, target_samples=1 << 19,
+                                 calls=2)
+    assert again == (best, ns)  # cached

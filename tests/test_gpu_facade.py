@@ -1,0 +1,67 @@
+"""The C++ drop-in facade (include/msosm/*.h), driven by a C++ program the
+way the reference's drivers drive its classes, against the oracle."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "tests", "cpp", "facade_check")
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def test_facade_program(device, tmp_path):
+    if not os.path.exists(BIN):
+        pytest.fail("tests/cpp/facade_check is not built (run __graft_entry__.build())")
+    rng = np.random.Generator(np.random.Philox(77))
+    L = 1 << 15
+    u = np.clip(np.round(32768 + 900 * rng.standard_normal(2 * L)), 1, 65535).astype(np.uint16)
+    inp = tmp_path / "in.u16"
+    u.tofile(inp)
+    pre = str(tmp_path / "out")
+    r = subprocess.run([BIN, str(inp), str(L), pre], capture_output=True, text=True, timeout=240)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0, r.stderr
+    meta = open(pre + ".meta").read()
+    assert "batch M 512 M_min 256 Nd 7743 K 16 count 8" in meta
+    bw, f0, M = 16e6, 1392e6, 512
+    x = oracle.unpack_u16(u).astype(np.complex128)
+
+    def ref(dm, M=M, osm=False):
+        d = np.zeros(2 * M, np.int32) if osm else oracle.delay_points(bw, dm, f0, M)[0]
+        return oracle.msosm_run(x, M, d, oracle.dedisp_params(bw, dm, f0, M))
+
+    y = np.fromfile(pre + ".batch.c64", np.complex64)
+    assert rel_l2(y, ref(10.0)) < 1e-5
+    conc = np.fromfile(pre + ".conc.c64", np.complex64).reshape(-1, 3, 8 * M)
+    for i, dm in enumerate((5.0, 10.0, 20.0)):
+        assert rel_l2(conc[:, i].reshape(-1), ref(dm)) < 1e-5
+    cu = np.fromfile(pre + ".conc.u16", np.uint16)
+    np.testing.assert_array_equal(cu, oracle.pack_u16(conc.reshape(-1)))
+    loop = np.fromfile(pre + ".loop.c64", np.complex64).reshape(-1, 3, 8 * M)
+    np.testing.assert_array_equal(loop, conc)
+    osm_line = [ln for ln in meta.splitlines() if ln.startswith("osm")][0].split()
+    Mo, co = int(osm_line[2]), int(osm_line[4])
+    yo = np.fromfile(pre + ".osm.c64", np.complex64)
+    n = yo.size
+    assert n == (L // (co * Mo)) * co * Mo
+    assert rel_l2(yo, ref(2.0, M=Mo, osm=True)[:n]) < 1e-5
+    # fold chain: intensity of (y, y) on the GPU, then the fold, bit for bit
+    inten = np.fromfile(pre + ".inten.f32", np.float32)
+    np.testing.assert_array_equal(inten, oracle.intensity(y, y))
+    f = oracle.Fold(0.0005, bw, 8 * M, 100)
+    f.discard_samples(7743)
+    for c in range(L // (8 * M)):
+        f.fold_data(inten[c * 8 * M:(c + 1) * 8 * M])
+    f.bins()
+    np.testing.assert_array_equal(np.fromfile(pre + ".fold.f32", np.float32), f.folded())
+    txt = open(pre + ".fold.txt").read().split()
+    assert len(txt) == 100

@@ -75,3 +75,29 @@ def test_intensity_then_fold_end_to_end(device):
     e1.close()
     e2.close()
     fold.close()
+
+
+def test_phase_fold_lds_histogram(device):
+    # FP64 phase per sample, LDS histogram per workgroup: bin assignment
+    # (hit counts) exact, sums to float tolerance, for 3 trials at once
+    import torch
+    rng = np.random.default_rng(9)
+    ndm, n, stride, nbins, P, n0 = 3, 100000, 100352, 64, 1234.567, 5000
+    a = (rng.standard_normal((ndm, stride)) + 1j * rng.standard_normal((ndm, stride))).astype(np.complex64)
+    b = (rng.standard_normal((ndm, stride)) + 1j * rng.standard_normal((ndm, stride))).astype(np.complex64)
+    ad = torch.from_numpy(a).to(device)
+    bd = torch.from_numpy(b).to(device)
+    for power in (True, False):
+        prof = torch.zeros((ndm, nbins), dtype=torch.float32, device=device)
+        hits = torch.zeros(nbins, dtype=torch.float32, device=device)
+        ms.fold_phase(ad, n, P, nbins, prof, hits, b=bd, ndm=ndm, stride=stride, n0=n0, power=power)
+        torch.cuda.synchronize()
+        ph = (np.arange(n, dtype=np.float64) + n0) * (1.0 / P)
+        ph -= np.floor(ph)
+        bins = np.minimum((ph * nbins).astype(np.int64), nbins - 1)
+        np.testing.assert_array_equal(hits.cpu().numpy(), np.bincount(bins, minlength=nbins).astype(np.float32))
+        v = np.abs(a[:, :n].astype(np.complex128)) ** 2 + np.abs(b[:, :n].astype(np.complex128)) ** 2
+        if not power:
+            v = np.sqrt(v)
+        exp = np.stack([np.bincount(bins, weights=v[d], minlength=nbins) for d in range(ndm)])
+        assert np.max(np.abs(prof.cpu().numpy() - exp) / exp) < 1e-5
