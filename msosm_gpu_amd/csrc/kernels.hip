@@ -502,14 +502,13 @@ struct RingElem<false> {
 // items are ordered so that each XCD (workgroups are dealt to the 8 XCDs
 // round robin) gets a contiguous range of DM trials and keeps their tables in
 // its L2.
-template <int LOG2N, bool F16, int NB, bool PREFETCH>
-__global__ __launch_bounds__(fft_threads(LOG2N)) void k_inv_fused_nb(
+template <int LOG2N, int E, bool F16, int NB, bool PREFETCH>
+__global__ __launch_bounds__((1 << LOG2N) / E) void k_inv_fused_nb(
     const void *__restrict__ ring, const void *__restrict__ H, const uint32_t *__restrict__ dpk,
     float2 *__restrict__ out, int count, int ndm, long long g0, int rx)
 {
-    constexpr int E = fft_e(LOG2N);
     using S = FftShape<LOG2N, E>;
-    static_assert(fft_tpb(LOG2N) == 1, "one transform per workgroup");
+    static_assert(S::NT >= 256, "one transform per workgroup");
     constexpr long long N = S::N, M = S::N / 2;
     constexpr int NT = S::NT;
     using R = RingElem<F16>;
@@ -528,13 +527,17 @@ __global__ __launch_bounds__(fft_threads(LOG2N)) void k_inv_fused_nb(
     const int dm = w / chunks;
     const int b0 = (w - dm * chunks) * NB;
     const int nb = min(NB, count - b0);
-    const T *ringT = reinterpret_cast<const T *>(ring);
+    const char *ringB = reinterpret_cast<const char *>(ring);
     const T *hrow = reinterpret_cast<const T *>(H) + (long long)dm * N;
     const uint32_t *drow = dpk + (long long)dm * N / 2;
     long long g = g0 + b0;
     int slot = (int)(g % rx);
+    // blocks before the stream start read as zero; delays are < 65536, so
+    // a clamped 32-bit copy of g is enough for the comparison
+    int gc = (int)(g < 65536 ? g : 65536);
     T raw[E];
-    // ring offsets fit 32 bits (rx * N elements): one VGPR per address
+    // byte offsets fit 32 bits (the ring is < 4 GiB): loads use a scalar base
+    // plus one 32-bit VGPR offset
     auto gather = [&]() {
 #pragma unroll
         for (int j = 0; j < E / 2; j++) {
@@ -542,11 +545,12 @@ __global__ __launch_bounds__(fft_threads(LOG2N)) void k_inv_fused_nb(
 #pragma unroll
             for (int q = 0; q < 2; q++) {
                 const int m = 2 * j + q;
-                const int d = (int)((pair >> (16 * q)) & 0xFFFFu);
+                const int d = q ? (int)(pair >> 16) : (int)(pair & 0xFFFFu);
                 int sl = slot - d;
                 sl += sl < 0 ? rx : 0;
-                const uint32_t off = (uint32_t)sl * (uint32_t)N + (uint32_t)(t + m * NT);
-                raw[m] = g >= d ? ringT[off] : R::zero();
+                const uint32_t off =
+                    ((uint32_t)sl * (uint32_t)N + (uint32_t)(t + m * NT)) * (uint32_t)sizeof(T);
+                raw[m] = gc >= d ? *reinterpret_cast<const T *>(ringB + off) : R::zero();
             }
         }
     };
@@ -562,12 +566,12 @@ __global__ __launch_bounds__(fft_threads(LOG2N)) void k_inv_fused_nb(
             v[m] = cmul(R::f2(raw[m]), R::f2(hrow[t + m * NT]));
         if constexpr (PREFETCH) {
             if (i + 1 < nb) {
-                g += 1;
+                gc += gc < 65536 ? 1 : 0;
                 slot = slot + 1 == rx ? 0 : slot + 1;
                 gather();
             }
         } else {
-            g += 1;
+            gc += gc < 65536 ? 1 : 0;
             slot = slot + 1 == rx ? 0 : slot + 1;
         }
         fft_block<LOG2N, E, true>(v, smem, twlo, twhi, t);
@@ -738,27 +742,30 @@ constexpr int kInvBlocksPerWg = 8;
 
 // Inverse-kernel variant for the sizes with one transform per workgroup:
 // 0 = one (DM, block) per workgroup, 1 = kInvBlocksPerWg blocks per
-// workgroup, 2 = the same with the next block's gather prefetched (default).
-// MSOSM_INV_KERNEL overrides the choice (tuning runs).
+// workgroup (default), 2 = as 1 with the next block's gather prefetched into
+// registers, 3 = as 2 with 16 points per thread for 2^14 (1024 threads).
+// Measured on MI355X for 256 trials x 64 blocks of 2^14 points (FP16 ring):
+// 2.13 / 0.66 / 1.75 / 1.75 ms per launch -- the prefetch costs register
+// spills that outweigh the overlap.  MSOSM_INV_KERNEL overrides the choice.
 static int inv_variant()
 {
     static int v = [] {
         const char *e = getenv("MSOSM_INV_KERNEL");
-        int x = e ? atoi(e) : 2;
-        return (x >= 0 && x <= 2) ? x : 2;
+        int x = e ? atoi(e) : 1;
+        return (x >= 0 && x <= 3) ? x : 1;
     }();
     return v;
 }
 
-template <int LOG2N, bool F16, bool PF>
+template <int LOG2N, int E, bool F16, bool PF>
 static void launch_nb(const void *ring, const void *H, const uint32_t *dpk, float2 *out, int count, int ndm,
                       long long g0, int rx, hipStream_t s)
 {
-    constexpr int threads = fft_threads(LOG2N);
+    constexpr int threads = (1 << LOG2N) / E;
     const long long chunks = (count + kInvBlocksPerWg - 1) / kInvBlocksPerWg;
     const long long grid = chunks * ndm;
-    hipLaunchKernelGGL((k_inv_fused_nb<LOG2N, F16, kInvBlocksPerWg, PF>), dim3((unsigned)grid), dim3(threads), 0, s,
-                       ring, H, dpk, out, count, ndm, g0, rx);
+    hipLaunchKernelGGL((k_inv_fused_nb<LOG2N, E, F16, kInvBlocksPerWg, PF>), dim3((unsigned)grid), dim3(threads), 0,
+                       s, ring, H, dpk, out, count, ndm, g0, rx);
 }
 
 template <int LOG2N>
@@ -770,16 +777,21 @@ static hipError_t inv_fused(const void *ring, const void *H, const uint16_t *del
     const int var = inv_variant();
     if (fft_tpb(LOG2N) == 1 && var > 0) {
         if constexpr (fft_tpb(LOG2N) == 1) {
-            if (f16) {
-                if (var == 2)
-                    launch_nb<LOG2N, true, true>(ring, H, dpk, out, count, ndm, g0, rx, s);
+            if (LOG2N == 14 && var == 3) {
+                if (f16)
+                    launch_nb<LOG2N, 16, true, true>(ring, H, dpk, out, count, ndm, g0, rx, s);
                 else
-                    launch_nb<LOG2N, true, false>(ring, H, dpk, out, count, ndm, g0, rx, s);
+                    launch_nb<LOG2N, 16, false, true>(ring, H, dpk, out, count, ndm, g0, rx, s);
+            } else if (f16) {
+                if (var >= 2)
+                    launch_nb<LOG2N, E, true, true>(ring, H, dpk, out, count, ndm, g0, rx, s);
+                else
+                    launch_nb<LOG2N, E, true, false>(ring, H, dpk, out, count, ndm, g0, rx, s);
             } else {
-                if (var == 2)
-                    launch_nb<LOG2N, false, true>(ring, H, dpk, out, count, ndm, g0, rx, s);
+                if (var >= 2)
+                    launch_nb<LOG2N, E, false, true>(ring, H, dpk, out, count, ndm, g0, rx, s);
                 else
-                    launch_nb<LOG2N, false, false>(ring, H, dpk, out, count, ndm, g0, rx, s);
+                    launch_nb<LOG2N, E, false, false>(ring, H, dpk, out, count, ndm, g0, rx, s);
             }
         }
     } else {
@@ -822,7 +834,12 @@ hipError_t launch_fwd_u16(const uint32_t *cur, const uint32_t *tail, void *ring,
 #undef CALL_F
 }
 
-int fused_delay_threads(int log2n) { return (1 << log2n) / fft_e(log2n); }
+int fused_delay_threads(int log2n)
+{
+    if (log2n == 14 && inv_variant() == 3)
+        return (1 << log2n) / 16;
+    return (1 << log2n) / fft_e(log2n);
+}
 bool fused_uses_packed_delay(int log2n) { return log2n >= 2 && log2n <= kMaxLog2SinglePass && fft_tpb(log2n) == 1; }
 
 hipError_t launch_inv_fused(const void *ring, const void *H, const uint16_t *delay, const uint32_t *dpk, float2 *out,
