@@ -226,6 +226,48 @@ int msosm_autotune_fftpoint(const msosm_plan_desc *desc, unsigned long min_fftpo
                             unsigned long target_samples, int calls, void *stream, unsigned long *best_fftpoint,
                             double *best_ns_per_dm_sample);
 
+/* ---- learned DM proposal (north-star component; no reference counterpart:
+ * the reference dedisperses every trial of a grid,
+ * test_processing_speed_msosm_dmtrial.cpp) -------------------------------
+ * Device pointers throughout; activations are FP32 [B][C][T], convolution
+ * weights FP16 [Cout][Cin][KW], linear weights FP16 [O][I]. */
+/* Synthetic pulsar without noise: x[i] = sqrt(amp g(phase_i)) z_i, z ~ CN(0,1),
+ * Gaussian pulse of FWHM duty (fraction of the period), phase in FP64. */
+int msosm_sim_pulsar(void *x_c64, size_t n, double period_samples, double duty, float amp, unsigned int seed,
+                     void *stream);
+/* X[b] *= exp(sign i phi(f_b)) over the bins of an n-point spectrum. */
+int msosm_sim_disperse(void *X_c64, size_t n, double fs, double f0, double dm, int sign, void *stream);
+/* u16 pairs = clip(round(scale (x x_scale + CN(0,1) noise)) + 32768, 1, 65535). */
+int msosm_sim_quantize(const void *x_c64, size_t n, float x_scale, unsigned int seed, float scale, void *u16_out,
+                       void *stream);
+/* Power spectrogram out[nchan][T], T = n / (nchan navg): navg consecutive
+ * nchan-point spectra per bin (this build: nchan 256, navg 16), channel 0 =
+ * lowest frequency. */
+int msosm_spectrogram_u16(const void *u16, size_t n, int nchan, int navg, float *out, void *stream);
+int msosm_rows_standardize(float *x, int rows, int T, void *stream);
+/* out[k][t] = sum_c S[c][t + delay[k][c]] / sqrt(nchan), t < Tout. */
+int msosm_incoherent_dedisperse(const float *S, int nchan, int T, const int *delay, int K, int Tout, float *out,
+                                void *stream);
+int msosm_conv1d_fwd(const float *x, const void *w16, const float *bias, float *y, int B, int Cin, int Cout, int T,
+                     int KW, int dil, int relu, void *stream);
+/* gx (may be NULL) = data gradient; gw, gb are accumulated (+=). */
+int msosm_conv1d_bwd(const float *x, const void *w16, const float *y, const float *gy, float *gx, float *gw,
+                     float *gb, int B, int Cin, int Cout, int T, int KW, int dil, int relu, void *stream);
+/* feat[B][2C] = (max_t, mean_t); amax[B][C] = argmax used by the backward. */
+int msosm_pool_fwd(const float *x, int B, int C, int T, float *feat, int *amax, void *stream);
+int msosm_pool_bwd(const float *gfeat, const int *amax, int B, int C, int T, float *gx, void *stream);
+int msosm_linear_fwd(const float *x, const void *w16, const float *bias, float *y, int B, int I, int O, void *stream);
+int msosm_linear_bwd(const float *x, const void *w16, const float *gy, float *gx, float *gw, float *gb, int B, int I,
+                     int O, void *stream);
+/* loss (accumulated, +=) = mean_b cross-entropy(softmax(logits_b), target_b);
+ * glogits = (softmax - target) / B. */
+int msosm_softmax_xent(const float *logits, const float *target, int B, int O, float *loss, float *glogits,
+                       void *stream);
+/* Adam step on FP32 master weights p, refreshing the FP16 copy p16 (may be NULL). */
+int msosm_adam(float *p, void *p16, const float *g, float *m, float *v, size_t n, float lr, float b1, float b2,
+               float eps, int step, void *stream);
+int msosm_f32_to_f16(const float *a, void *b, size_t n, void *stream);
+
 const char *msosm_last_error(void);
 const char *msosm_version(void);
 

@@ -110,6 +110,21 @@ SIGNATURES = {
     "msosm_batch_destroy": (_i, [_vp]),
     "msosm_autotune_fftpoint": (_i, [ctypes.POINTER(PlanDesc), _ul, _ul, _ul, _i, _vp, ctypes.POINTER(_ul),
                                      ctypes.POINTER(ctypes.c_double)]),
+    "msosm_sim_pulsar": (_i, [_vp, _sz, ctypes.c_double, ctypes.c_double, _f, ctypes.c_uint, _vp]),
+    "msosm_sim_disperse": (_i, [_vp, _sz, ctypes.c_double, ctypes.c_double, ctypes.c_double, _i, _vp]),
+    "msosm_sim_quantize": (_i, [_vp, _sz, _f, ctypes.c_uint, _f, _vp, _vp]),
+    "msosm_spectrogram_u16": (_i, [_vp, _sz, _i, _i, _vp, _vp]),
+    "msosm_rows_standardize": (_i, [_vp, _i, _i, _vp]),
+    "msosm_incoherent_dedisperse": (_i, [_vp, _i, _i, _vp, _i, _i, _vp, _vp]),
+    "msosm_conv1d_fwd": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _i, _i, _vp]),
+    "msosm_conv1d_bwd": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _i, _i, _vp]),
+    "msosm_pool_fwd": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp]),
+    "msosm_pool_bwd": (_i, [_vp, _vp, _i, _i, _i, _vp, _vp]),
+    "msosm_linear_fwd": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _vp]),
+    "msosm_linear_bwd": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _vp]),
+    "msosm_softmax_xent": (_i, [_vp, _vp, _i, _i, _vp, _vp, _vp]),
+    "msosm_adam": (_i, [_vp, _vp, _vp, _vp, _vp, _sz, _f, _f, _f, _f, _i, _vp]),
+    "msosm_f32_to_f16": (_i, [_vp, _vp, _sz, _vp]),
     "msosm_last_error": (ctypes.c_char_p, []),
     "msosm_version": (ctypes.c_char_p, []),
 }
