@@ -42,7 +42,7 @@ def parse():
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP32 comparison")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-check", action="store_true", help="skip the FP64 spot check")
-    ap.add_argument("--cpu-log2-samples", type=int, default=22)
+    ap.add_argument("--cpu-log2-samples", type=int, default=25)
     return ap.parse_args()
 
 

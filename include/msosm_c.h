@@ -48,6 +48,12 @@ extern "C" {
                                /* the inverse FFT, discard fused (FP32)       */
 #define MSOSM_FORM_FUSED_F16 2 /* as FUSED_F32 with FP16 spectra ring and H   */
                                /* (pre-scaled), FP32 arithmetic               */
+/* Delay-line accumulate as per-bin complex GEMMs on the matrix cores       */
+/* (Y_n = A_n B_n, A_n one H per row at the segment delay, B_n the Hankel  */
+/* block of ring spectra), then a separate inverse FFT of the products:     */
+#define MSOSM_FORM_HANKEL_F16 3    /* FP16 operands, FP32 accumulate         */
+#define MSOSM_FORM_HANKEL_BF16 4   /* BF16 operands (single), FP32 accumulate */
+#define MSOSM_FORM_HANKEL_BF16X3 5 /* BF16 hi+lo split operands, 3 products  */
 
 typedef struct msosm_plan_desc {
     int num_dms;             /* DM trials sharing one input stream (>=1)      */

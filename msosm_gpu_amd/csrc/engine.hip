@@ -107,6 +107,10 @@ struct msosm_engine {
     uint32_t *tail = nullptr;       // previous call's last M input samples
     void *xring = nullptr;          // rx*2M spectra (float2 or half2)
     int rx = 0;
+    // HANKEL (delay-line accumulate as per-bin GEMMs on MFMA)
+    void *ybuf = nullptr;           // [ndm][count][2M] products (half2 or float2)
+    int hankel_mode = -1;
+    int kmax = 0;                   // largest delay + 1 over all trials
     long long g0 = 0;               // global index of the next block
     bool have_output = false;
     // timing
@@ -184,7 +188,7 @@ void free_engine(msosm_engine *e)
     for (auto ev : e->pool)
         hipEventDestroy(ev);
     void *bufs[] = {e->in_stage, e->H32, e->H16, e->delay32, e->delay16, e->delay_pk, e->out, e->out_u16, e->inbuf,
-                    e->fftblk, e->ring, e->ifftblk, e->fft_scratch, e->tail, e->xring};
+                    e->fftblk, e->ring, e->ifftblk, e->fft_scratch, e->tail, e->xring, e->ybuf};
     for (void *b : bufs)
         if (b)
             hipFree(b);
@@ -261,6 +265,17 @@ int build_filter_tables(msosm_engine *e)
     return rc;
 }
 
+bool form_f16_ring(int form) { return form == MSOSM_FORM_FUSED_F16 || form == MSOSM_FORM_HANKEL_F16; }
+bool form_hankel(int form)
+{
+    return form == MSOSM_FORM_HANKEL_F16 || form == MSOSM_FORM_HANKEL_BF16 || form == MSOSM_FORM_HANKEL_BF16X3;
+}
+int hankel_mode_of(int form)
+{
+    return form == MSOSM_FORM_HANKEL_F16 ? msosm::HANKEL_F16
+           : form == MSOSM_FORM_HANKEL_BF16 ? msosm::HANKEL_BF16 : msosm::HANKEL_BF16X3;
+}
+
 int create_engine(const msosm::Plan &plan, void *stream, msosm_engine **out)
 {
     msosm_engine *e = new msosm_engine();
@@ -277,6 +292,21 @@ int create_engine(const msosm::Plan &plan, void *stream, msosm_engine **out)
     if (e->form != MSOSM_FORM_PLAIN_F32 && !msosm::fused_supported(e->log2n))
         return bail(set_error(MSOSM_ERR_UNSUPPORTED, "fused forms support FFT sizes 4..16384 (got %lu); use "
                                                      "MSOSM_FORM_PLAIN_F32", N));
+    if (form_hankel(e->form)) {
+        e->hankel_mode = hankel_mode_of(e->form);
+        int km = 0;
+        for (int32_t d : p.delay)
+            km = std::max(km, (int)d + 1);
+        e->kmax = km;
+        if (N % (unsigned long)msosm::hankel_bins_per_wg(e->hankel_mode) != 0)
+            return bail(set_error(MSOSM_ERR_UNSUPPORTED, "Hankel forms need an FFT size that is a multiple of %d",
+                                  msosm::hankel_bins_per_wg(e->hankel_mode)));
+        if (km > msosm::hankel_max_kmax(e->hankel_mode))
+            return bail(set_error(MSOSM_ERR_UNSUPPORTED,
+                                  "Hankel forms hold up to %d delay segments per bin in LDS (plan has %d); use a "
+                                  "larger FFT size or a fused form",
+                                  msosm::hankel_max_kmax(e->hankel_mode), km));
+    }
     int ndev = 0;
     hipError_t herr = hipGetDeviceCount(&ndev);
     if (herr != hipSuccess || ndev < 1)
@@ -316,8 +346,19 @@ int create_engine(const msosm::Plan &plan, void *stream, msosm_engine **out)
         HIP_TRY(hipMemcpyAsync(e->delay32, p.delay.data(), sizeof(int32_t) * ndm * N, hipMemcpyHostToDevice,
                                e->stream));
     } else {
-        const bool f16 = e->form == MSOSM_FORM_FUSED_F16;
+        const bool f16 = form_f16_ring(e->form);
         e->rx = p.K_max - 1 + count;
+        if (form_hankel(e->form)) {
+            if (f16) {
+                __half2 *y = nullptr;
+                ALLOC(y, (size_t)ndm * count * N);
+                e->ybuf = y;
+            } else {
+                float2 *y = nullptr;
+                ALLOC(y, (size_t)ndm * count * N);
+                e->ybuf = y;
+            }
+        }
         ALLOC(e->delay16, (size_t)ndm * N);
         ALLOC(e->tail, M);
         if (f16) {
@@ -342,7 +383,7 @@ int create_engine(const msosm::Plan &plan, void *stream, msosm_engine **out)
         (void)hipFree(tmp);
         if (herr != hipSuccess)
             return bail(set_error(MSOSM_ERR_HIP, "delay table upload: %s", hipGetErrorString(herr)));
-        if (msosm::fused_uses_packed_delay(e->log2n)) {
+        if (!form_hankel(e->form) && msosm::fused_uses_packed_delay(e->log2n)) {
             const int NT = msosm::fused_delay_threads(e->log2n);
             const int E = (int)(N / NT);
             std::vector<uint32_t> pk((size_t)ndm * N / 2);
@@ -408,6 +449,40 @@ int filter_plain(msosm_engine *e, const uint32_t *cur)
     return MSOSM_OK;
 }
 
+// Hankel forms: shared forward spectra as in the fused forms, the
+// delay-line accumulate as per-bin GEMMs on MFMA into dense products, then
+// the inverse transform with the discard fused.
+int filter_hankel(msosm_engine *e, const uint32_t *cur)
+{
+    const msosm::Plan &p = e->plan;
+    const unsigned long M = p.M, N = p.N;
+    const int ndm = p.num_dms, count = p.count;
+    const bool f16 = form_f16_ring(e->form);
+    hipStream_t s = e->stream;
+    {
+        StageScope st(e, MSOSM_STAGE_FFT_FWD);
+        HIP_TRY(msosm::launch_fwd_u16(cur, e->tail, e->xring, f16, e->log2n, count, e->g0, e->rx, s));
+    }
+    {
+        StageScope st(e, MSOSM_STAGE_COPY);
+        HIP_TRY(msosm::launch_copy_u32(e->tail, cur + (size_t)(count - 1) * M, M, s));
+    }
+    {
+        StageScope st(e, MSOSM_STAGE_MULTIPLY);
+        HIP_TRY(msosm::launch_hankel(e->xring, f16 ? (const void *)e->H16 : (const void *)e->H32, e->delay16, e->ybuf,
+                                     e->hankel_mode, e->log2n, count, ndm, e->g0, e->rx, e->kmax, s));
+    }
+    {
+        StageScope st(e, MSOSM_STAGE_FFT_INV);
+        // FP16 products are (X/N)(H/M)/2, FP32 products X H: both give
+        // IDFT(X H) / N^2 after these scales
+        const float scale = f16 ? 1.0f : 1.0f / ((float)N * (float)N);
+        HIP_TRY(msosm::launch_inv_dense(e->ybuf, e->out, f16, e->log2n, (long long)ndm * count, scale, s));
+    }
+    e->g0 += count;
+    return MSOSM_OK;
+}
+
 int filter_fused(msosm_engine *e, const uint32_t *cur)
 {
     const msosm::Plan &p = e->plan;
@@ -443,7 +518,7 @@ int msosm::engine_reset_async(msosm_engine *e)
         HIP_TRY(msosm::launch_fill_u32(e->ring, 2 * (size_t)p.num_dms * p.R * p.N, 0, e->stream));
         e->current_index = 0;
     } else {
-        size_t words = (e->form == MSOSM_FORM_FUSED_F16 ? 1 : 2) * (size_t)e->rx * p.N;
+        size_t words = (form_f16_ring(e->form) ? 1 : 2) * (size_t)e->rx * p.N;
         HIP_TRY(msosm::launch_fill_u32(e->xring, words, 0, e->stream));
         HIP_TRY(msosm::launch_fill_u32(e->tail, p.M, 0, e->stream));
         e->g0 = 0;
@@ -563,7 +638,9 @@ int msosm_engine_filter_u16(msosm_engine *e, const void *input, int where)
     } else if (where != 1) {
         return set_error(MSOSM_ERR_ARG, "filter_u16: where must be 0 (host) or 1 (device)");
     }
-    int rc = (e->form == MSOSM_FORM_PLAIN_F32) ? filter_plain(e, cur) : filter_fused(e, cur);
+    int rc = (e->form == MSOSM_FORM_PLAIN_F32) ? filter_plain(e, cur)
+             : form_hankel(e->form)             ? filter_hankel(e, cur)
+                                                : filter_fused(e, cur);
     if (rc == MSOSM_OK)
         e->have_output = true;
     return rc;

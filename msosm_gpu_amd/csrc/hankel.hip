@@ -1,0 +1,405 @@
+// hankel.hip -- the delay-line accumulate recast as per-bin complex GEMMs on
+// the gfx950 matrix cores (MFMA), plus the inverse transform that consumes
+// their dense output.
+//
+// For frequency bin n, with X_a[n] the ring of forward spectra (block a) and
+// d_t(n) the segment delay of DM trial t, the reference's multiply-delay
+// (gpu_kernel.cu:84-100, 176-196) writes Y_t,g[n] = H_t[n] X_{g-d_t(n)}[n]
+// for every output block g.  As a product of matrices:
+//
+//     Y_n[t][g] = sum_k A_n[t][k] B_n[k][g],   A_n[t][k] = H_t[n] [k == d_t(n)],
+//                                            B_n[k][g] = X_{g-k}[n]  (Hankel in g, k)
+//
+// A_n has one non-zero per row, so most MACs of a dense product are wasted.
+// The kernel runs the product over the band of k that a tile of 16 trials
+// actually uses at bin n ([min_t d_t(n), max_t d_t(n)], one or two 16-wide
+// k steps when the trials of a tile are close in DM) and skips the rest, so
+// issued MACs stay within a small factor of the useful ones.
+//
+// Complex arithmetic uses the real embedding along k:
+//     Re Y = [Re A | -Im A] [Re B ; Im B],   Im Y = [Im A | Re A] [Re B ; Im B]
+// so one 32-deep MFMA step covers 16 complex k.  Operand forms:
+//   MODE_F16     FP16 ring X/N and FP16 H/M, v_mfma_f32_16x16x32_f16,
+//                Y stored FP16 as (X/N)(H/M)/2
+//   MODE_BF16    FP32 ring X and FP32 H rounded to BF16, one product,
+//                Y stored FP32 as X H
+//   MODE_BF16X3  as MODE_BF16 with split operands x = hi + lo (both BF16)
+//                and three products hi*hi + hi*lo + lo*hi (FP32 accumulate)
+//
+// MFMA 16x16x32 operand layout (64 lanes, lane l, element j = 0..7):
+//   A[i][k]: i = l % 16, k = 8 (l / 16) + j
+//   B[k][c]: c = l % 16, k = 8 (l / 16) + j
+//   C[i][c] (4 floats): c = l % 16, i = 4 (l / 16) + j
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+
+#include <climits>
+#include <cstdint>
+
+#include "fft_device.h"
+#include "kernels.h"
+
+namespace msosm {
+
+namespace {
+
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kDmTile = 16; // trials per MFMA tile (rows)
+constexpr int kGTile = 16;  // output blocks per workgroup (columns)
+constexpr int kWaves = 4;
+
+template <int MODE>
+struct HankelMode;
+
+template <>
+struct HankelMode<HANKEL_F16> {
+    using RingT = __half2;
+    using YT = __half2;
+    using Op = _Float16;
+    using OpV = f16x8;
+    static constexpr int NBIN = 32; // 32 bins x 4 B = one 128 B line per output row
+    static constexpr int TERMS = 1;
+    static constexpr bool SPLIT = false;
+    __device__ static float2 load(const RingT *p) { return __half22float2(*p); }
+    __device__ static f32x4 mma(OpV a, OpV b, f32x4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0); }
+    __device__ static YT out(float re, float im) { return __floats2half2_rn(0.5f * re, 0.5f * im); }
+};
+
+template <>
+struct HankelMode<HANKEL_BF16> {
+    using RingT = float2;
+    using YT = float2;
+    using Op = __bf16;
+    using OpV = bf16x8;
+    static constexpr int NBIN = 16; // 16 bins x 8 B = one 128 B line per output row
+    static constexpr int TERMS = 1;
+    static constexpr bool SPLIT = false;
+    __device__ static float2 load(const RingT *p) { return *p; }
+    __device__ static f32x4 mma(OpV a, OpV b, f32x4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
+    __device__ static YT out(float re, float im) { return make_float2(re, im); }
+};
+
+template <>
+struct HankelMode<HANKEL_BF16X3> : HankelMode<HANKEL_BF16> {
+    static constexpr int TERMS = 3;
+    static constexpr bool SPLIT = true;
+};
+
+// Operand parts kept in LDS per (bin, u): real and imaginary, and for the
+// split modes their low parts.
+template <int MODE>
+struct Parts {
+    static constexpr int P = HankelMode<MODE>::SPLIT ? 4 : 2;
+};
+
+template <typename Op>
+__device__ __forceinline__ Op to_op(float x)
+{
+    return (Op)x;
+}
+
+template <typename Op>
+__device__ __forceinline__ float from_op(Op x)
+{
+    return (float)x;
+}
+
+template <int MODE>
+__device__ __forceinline__ void split(float x, typename HankelMode<MODE>::Op &hi, typename HankelMode<MODE>::Op &lo)
+{
+    using Op = typename HankelMode<MODE>::Op;
+    hi = to_op<Op>(x);
+    lo = HankelMode<MODE>::SPLIT ? to_op<Op>(x - from_op<Op>(hi)) : to_op<Op>(0.f);
+}
+
+// One workgroup: NBIN consecutive bins x kGTile consecutive output blocks,
+// looping over all DM tiles.  LDS (dynamic):
+//   z[P][NBIN][U]   ring column parts, u = kGTile-1 - (g - gb) + k
+//   dd[kDmTile][NBIN], hh[kDmTile][NBIN] (float2)   one DM tile's tables
+//   ys[kDmTile][kGTile][NBIN+1]                      staged output
+template <int MODE>
+__global__ __launch_bounds__(64 * kWaves) void k_hankel(const void *__restrict__ ring_, const void *__restrict__ H_,
+                                                        const uint16_t *__restrict__ delay, void *__restrict__ Y_,
+                                                        int N, int count, int ndm, long long g0, int rx, int kmax,
+                                                        int U)
+{
+    using Md = HankelMode<MODE>;
+    using RingT = typename Md::RingT;
+    using YT = typename Md::YT;
+    using Op = typename Md::Op;
+    using OpV = typename Md::OpV;
+    constexpr int NBIN = Md::NBIN;
+    constexpr int P = Parts<MODE>::P;
+    const RingT *ring = reinterpret_cast<const RingT *>(ring_);
+    const RingT *H = reinterpret_cast<const RingT *>(H_);
+    YT *Y = reinterpret_cast<YT *>(Y_);
+
+    extern __shared__ __align__(16) unsigned char lds[];
+    Op *z = reinterpret_cast<Op *>(lds);                                  // [P][NBIN][U]
+    size_t off = ((size_t)P * NBIN * U * sizeof(Op) + 15) & ~(size_t)15;
+    int *dd = reinterpret_cast<int *>(lds + off);                         // [kDmTile][NBIN]
+    off += sizeof(int) * kDmTile * NBIN;
+    float2 *hh = reinterpret_cast<float2 *>(lds + off);                   // [kDmTile][NBIN]
+    off += sizeof(float2) * kDmTile * NBIN;
+    YT *ys = reinterpret_cast<YT *>(lds + off);                           // [kDmTile][kGTile][NBIN+1]
+    constexpr int YS_ROW = NBIN + 1;
+
+    const int n0 = blockIdx.x * NBIN;
+    const int gb = blockIdx.y * kGTile;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+
+    // ring columns of this workgroup's bins: u -> absolute block
+    // a = g0 + gb + kGTile - 1 - u; zero before the stream start and past the
+    // oldest block any (g, k) of the tile uses
+    for (int idx = tid; idx < NBIN * U; idx += blockDim.x) {
+        const int nl = idx % NBIN, u = idx / NBIN;
+        const long long a = g0 + gb + kGTile - 1 - u;
+        float2 x = make_float2(0.f, 0.f);
+        if (a >= 0 && u < kGTile - 1 + kmax) {
+            const long long slot = a % rx;
+            x = Md::load(ring + slot * N + n0 + nl);
+        }
+        Op rh, rl, ih, il;
+        split<MODE>(x.x, rh, rl);
+        split<MODE>(x.y, ih, il);
+        z[(0 * NBIN + nl) * U + u] = rh;
+        z[(1 * NBIN + nl) * U + u] = ih;
+        if constexpr (P == 4) {
+            z[(2 * NBIN + nl) * U + u] = rl;
+            z[(3 * NBIN + nl) * U + u] = il;
+        }
+    }
+
+    const int row = lane & 15;      // A row (trial in tile) / B column (block in tile)
+    const int q = lane >> 4;        // k group: 0,1 real rows, 2,3 imaginary rows
+    const int kq = 8 * (q & 1);
+    const bool imag_half = q >= 2;
+    const int ntiles = (ndm + kDmTile - 1) / kDmTile;
+    for (int tile = 0; tile < ntiles; tile++) {
+        __syncthreads(); // z ready (first tile); tables and staging free
+        for (int idx = tid; idx < kDmTile * NBIN; idx += blockDim.x) {
+            const int i = idx / NBIN, nl = idx % NBIN;
+            const int dm = tile * kDmTile + i;
+            int d = -1;
+            float2 h = make_float2(0.f, 0.f);
+            if (dm < ndm) {
+                d = delay[(long long)dm * N + n0 + nl];
+                h = Md::load(H + (long long)dm * N + n0 + nl);
+            }
+            dd[idx] = d;
+            hh[idx] = h;
+        }
+        __syncthreads();
+        for (int nl = wave; nl < NBIN; nl += kWaves) {
+            const int d = dd[row * NBIN + nl];
+            const float2 h = hh[row * NBIN + nl];
+            int lo = d >= 0 ? d : INT_MAX, hi = d;
+#pragma unroll
+            for (int m = 1; m < 16; m <<= 1) {
+                lo = min(lo, __shfl_xor(lo, m));
+                hi = max(hi, __shfl_xor(hi, m));
+            }
+            f32x4 cr = {0.f, 0.f, 0.f, 0.f}, ci = {0.f, 0.f, 0.f, 0.f};
+            // A entries: Re rows [Re h | -Im h], Im rows [Im h | Re h]
+            Op ar_h, ar_l, ai_h, ai_l;
+            split<MODE>(imag_half ? -h.y : h.x, ar_h, ar_l);
+            split<MODE>(imag_half ? h.x : h.y, ai_h, ai_l);
+            const Op zero = to_op<Op>(0.f);
+            const Op *zb = z + ((imag_half ? 1 : 0) * NBIN + nl) * U + (kGTile - 1 - row) + kq;
+            const Op *zl = z + ((imag_half ? 3 : 2) * NBIN + nl) * U + (kGTile - 1 - row) + kq;
+            for (int k0 = lo; k0 <= hi; k0 += 16) { // wave-uniform bounds
+                OpV a_r, a_i, b;
+#pragma unroll
+                for (int j = 0; j < 8; j++) {
+                    const bool hit = (k0 + kq + j) == d;
+                    a_r[j] = hit ? ar_h : zero;
+                    a_i[j] = hit ? ai_h : zero;
+                    b[j] = zb[k0 + j];
+                }
+                cr = Md::mma(a_r, b, cr);
+                ci = Md::mma(a_i, b, ci);
+                if constexpr (Md::TERMS == 3) {
+                    OpV al_r, al_i, bl;
+#pragma unroll
+                    for (int j = 0; j < 8; j++) {
+                        const bool hit = (k0 + kq + j) == d;
+                        al_r[j] = hit ? ar_l : zero;
+                        al_i[j] = hit ? ai_l : zero;
+                        bl[j] = zl[k0 + j];
+                    }
+                    cr = Md::mma(a_r, bl, cr);
+                    ci = Md::mma(a_i, bl, ci);
+                    cr = Md::mma(al_r, b, cr);
+                    ci = Md::mma(al_i, b, ci);
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int i = 4 * q + j; // trial in tile
+                ys[(i * kGTile + row) * YS_ROW + nl] = Md::out(cr[j], ci[j]);
+            }
+        }
+        __syncthreads();
+        // write the tile: rows (trial, block), NBIN consecutive bins each
+        for (int idx = tid; idx < kDmTile * kGTile * NBIN; idx += blockDim.x) {
+            const int nl = idx % NBIN, r = idx / NBIN;
+            const int i = r / kGTile, g = r % kGTile;
+            const int dm = tile * kDmTile + i;
+            if (dm < ndm && gb + g < count)
+                Y[((long long)dm * count + gb + g) * N + n0 + nl] = ys[r * YS_ROW + nl];
+        }
+    }
+}
+
+// Inverse transform of dense rows Y[dm][b][N] (one per (trial, block)), keep
+// the last M points, scale; out layout [dm][count][M].
+template <int LOG2N, bool F16Y>
+__global__ __launch_bounds__(fft_threads(LOG2N)) void k_inv_dense(const void *__restrict__ Y_, float2 *__restrict__ out,
+                                                                   long long units, float scale)
+{
+    constexpr int E = fft_e(LOG2N);
+    using S = FftShape<LOG2N, E>;
+    constexpr int TPB = fft_tpb(LOG2N);
+    constexpr long long N = S::N, M = S::N / 2;
+    __shared__ float2 smem[TPB * S::NPAD + S::TWLO + S::TWHI];
+    float2 *twlo = smem + TPB * S::NPAD, *twhi = twlo + S::TWLO;
+    if constexpr (S::NPASS > 1)
+        fill_twiddles<LOG2N, E>(twlo, twhi, threadIdx.x, blockDim.x);
+    const int tl = threadIdx.x / S::NT, t = threadIdx.x % S::NT;
+    const long long u = (long long)blockIdx.x * TPB + tl;
+    const bool valid = u < units;
+    float2 v[E];
+#pragma unroll
+    for (int m = 0; m < E; m++) {
+        const long long idx = u * N + t + m * S::NT;
+        if constexpr (F16Y)
+            v[m] = valid ? __half22float2(reinterpret_cast<const __half2 *>(Y_)[idx]) : make_float2(0.f, 0.f);
+        else
+            v[m] = valid ? reinterpret_cast<const float2 *>(Y_)[idx] : make_float2(0.f, 0.f);
+    }
+    fft_block<LOG2N, E, true>(v, smem + tl * S::NPAD, twlo, twhi, t);
+    if (valid) {
+        float2 *dst = out + u * M;
+#pragma unroll
+        for (int m = E / 2; m < E; m++)
+            dst[t + m * S::NT - M] = cscale(v[m], scale);
+    }
+}
+
+template <int MODE>
+size_t hankel_lds_bytes(int U)
+{
+    using Md = HankelMode<MODE>;
+    size_t b = ((size_t)Parts<MODE>::P * Md::NBIN * U * sizeof(typename Md::Op) + 15) & ~(size_t)15;
+    b += sizeof(int) * kDmTile * Md::NBIN + sizeof(float2) * kDmTile * Md::NBIN;
+    b += sizeof(typename Md::YT) * kDmTile * kGTile * (Md::NBIN + 1);
+    return b;
+}
+
+template <int MODE>
+hipError_t hankel_launch(const void *ring, const void *H, const uint16_t *delay, void *Y, int N, int count, int ndm,
+                         long long g0, int rx, int kmax, hipStream_t s)
+{
+    using Md = HankelMode<MODE>;
+    if (N % Md::NBIN != 0)
+        return hipErrorInvalidValue;
+    const int U = kGTile - 1 + kmax + 16;
+    const size_t lds = hankel_lds_bytes<MODE>(U);
+    if (lds > kHankelMaxLds)
+        return hipErrorInvalidValue;
+    static bool attr_set = false;
+    if (!attr_set) {
+        hipError_t e = hipFuncSetAttribute((const void *)k_hankel<MODE>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)kHankelMaxLds);
+        if (e != hipSuccess)
+            return e;
+        attr_set = true;
+    }
+    dim3 grid(N / Md::NBIN, (count + kGTile - 1) / kGTile);
+    hipLaunchKernelGGL(k_hankel<MODE>, grid, dim3(64 * kWaves), lds, s, ring, H, delay, Y, N, count, ndm, g0, rx,
+                       kmax, U);
+    return hipGetLastError();
+}
+
+template <int LOG2N>
+hipError_t inv_dense(const void *Y, float2 *out, bool f16y, long long units, float scale, hipStream_t s)
+{
+    constexpr int E = fft_e(LOG2N);
+    constexpr int threads = FftShape<LOG2N, E>::NT * fft_tpb(LOG2N);
+    const long long grid = (units + fft_tpb(LOG2N) - 1) / fft_tpb(LOG2N);
+    if (f16y)
+        hipLaunchKernelGGL((k_inv_dense<LOG2N, true>), dim3((unsigned)grid), dim3(threads), 0, s, Y, out, units,
+                           scale);
+    else
+        hipLaunchKernelGGL((k_inv_dense<LOG2N, false>), dim3((unsigned)grid), dim3(threads), 0, s, Y, out, units,
+                           scale);
+    return hipGetLastError();
+}
+
+} // namespace
+
+int hankel_max_kmax(int mode)
+{
+    // largest kmax whose LDS footprint fits kHankelMaxLds
+    int lo = 1, hi = 1 << 16;
+    auto fits = [&](int k) {
+        const int U = kGTile - 1 + k + 16;
+        size_t b = mode == HANKEL_F16 ? hankel_lds_bytes<HANKEL_F16>(U)
+                   : mode == HANKEL_BF16 ? hankel_lds_bytes<HANKEL_BF16>(U)
+                                         : hankel_lds_bytes<HANKEL_BF16X3>(U);
+        return b <= kHankelMaxLds;
+    };
+    if (!fits(lo))
+        return 0;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) / 2;
+        if (fits(mid))
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+int hankel_bins_per_wg(int mode) { return mode == HANKEL_F16 ? HankelMode<HANKEL_F16>::NBIN : HankelMode<HANKEL_BF16>::NBIN; }
+
+hipError_t launch_hankel(const void *ring, const void *H, const uint16_t *delay, void *Y, int mode, int log2n,
+                         int count, int ndm, long long g0, int rx, int kmax, hipStream_t s)
+{
+    const int N = 1 << log2n;
+    switch (mode) {
+    case HANKEL_F16: return hankel_launch<HANKEL_F16>(ring, H, delay, Y, N, count, ndm, g0, rx, kmax, s);
+    case HANKEL_BF16: return hankel_launch<HANKEL_BF16>(ring, H, delay, Y, N, count, ndm, g0, rx, kmax, s);
+    case HANKEL_BF16X3: return hankel_launch<HANKEL_BF16X3>(ring, H, delay, Y, N, count, ndm, g0, rx, kmax, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_inv_dense(const void *Y, float2 *out, bool f16y, int log2n, long long units, float scale,
+                            hipStream_t s)
+{
+#define CALL_D(L) inv_dense<L>(Y, out, f16y, units, scale, s)
+    switch (log2n) {
+    case 2: return CALL_D(2);
+    case 3: return CALL_D(3);
+    case 4: return CALL_D(4);
+    case 5: return CALL_D(5);
+    case 6: return CALL_D(6);
+    case 7: return CALL_D(7);
+    case 8: return CALL_D(8);
+    case 9: return CALL_D(9);
+    case 10: return CALL_D(10);
+    case 11: return CALL_D(11);
+    case 12: return CALL_D(12);
+    case 13: return CALL_D(13);
+    case 14: return CALL_D(14);
+    default: return hipErrorInvalidValue;
+    }
+#undef CALL_D
+}
+
+} // namespace msosm

@@ -32,6 +32,24 @@ int fused_delay_threads(int log2n);
 bool fused_uses_packed_delay(int log2n);
 bool fused_supported(int log2n);
 
+// Delay-line accumulate as per-bin complex GEMMs on MFMA (hankel.hip):
+// Y[dm][b][n] = H[dm][n] X_{g0+b-d(n)}[n] computed over the band of delays a
+// tile of 16 trials uses at bin n.  Modes: FP16 operands (ring X/N, H/M; Y
+// FP16 = product/2), BF16 operands from an FP32 ring and H (Y FP32 =
+// product), BF16 hi+lo split with three products.  kmax: largest delay + 1.
+constexpr int HANKEL_F16 = 0;
+constexpr int HANKEL_BF16 = 1;
+constexpr int HANKEL_BF16X3 = 2;
+constexpr size_t kHankelMaxLds = 80 * 1024;
+hipError_t launch_hankel(const void *ring, const void *H, const uint16_t *delay, void *Y, int mode, int log2n,
+                         int count, int ndm, long long g0, int rx, int kmax, hipStream_t s);
+int hankel_max_kmax(int mode);
+int hankel_bins_per_wg(int mode);
+// Inverse FFT of dense rows Y[unit][2^log2n] (FP16 or FP32 complex), keep the
+// last half, times scale: out[unit][2^(log2n-1)].
+hipError_t launch_inv_dense(const void *Y, float2 *out, bool f16y, int log2n, long long units, float scale,
+                            hipStream_t s);
+
 // Element kernels (reference gpu_kernel.cu semantics).
 hipError_t launch_unpack(float2 *dst, const uint32_t *src, size_t n, hipStream_t s);
 hipError_t launch_pack(uint32_t *dst, const float2 *src, size_t n, hipStream_t s);

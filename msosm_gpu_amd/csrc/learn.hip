@@ -185,7 +185,9 @@ __global__ void k_rows_standardize(float *__restrict__ x, int T)
         row[t] = (row[t] - mu) * inv;
 }
 
-// out[k][t] = sum_c S[c][t + delay[k][c]] / sqrt(nchan), t < Tout
+// out[k][t] = sum_c S[c][t + delay[k][c]] / sqrt(nchan), t < Tout; a
+// delayed index outside [0, T) contributes nothing (the host keeps every
+// delay within T - Tout, so this only guards against a bad table).
 __global__ void k_incoherent(const float *__restrict__ S, int nchan, int T, const int *__restrict__ delay, int Tout,
                              float *__restrict__ out)
 {
@@ -195,8 +197,11 @@ __global__ void k_incoherent(const float *__restrict__ S, int nchan, int T, cons
         return;
     const int *dk = delay + (size_t)k * nchan;
     float s = 0.f;
-    for (int c = 0; c < nchan; c++)
-        s += S[(size_t)c * T + t + dk[c]];
+    for (int c = 0; c < nchan; c++) {
+        const int ts = t + dk[c];
+        if ((unsigned)ts < (unsigned)T)
+            s += S[(size_t)c * T + ts];
+    }
     out[(size_t)k * Tout + t] = s * rsqrtf((float)nchan);
 }
 

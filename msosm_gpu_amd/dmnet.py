@@ -190,24 +190,91 @@ def simulate(n, bw, f0, dm, period_s, duty, snr, seed, device, scale=64.0):
     return u
 
 
-def folded_snr(u16_dev, n, bw, f0, dms, period_s, fftpoint=16384, count=256, nbins=64, device="cuda:0"):
-    """Plain coherent method: dedisperse u16 at every DM in dms (engine, FP32
-    fused form), phase-fold each call's output, return the folded S/N per DM."""
-    import torch
-    plan = Plan(bw, np.asarray(dms, np.float32), f0, fftpoint=fftpoint, count=count, form=FORM_FUSED_F32,
-                chirp=CHIRP_REF_F32W)
-    eng = Engine(plan, stream=torch.cuda.current_stream())
-    per = plan.count * plan.M
-    nd = len(dms)
-    prof = torch.zeros((nd, nbins), dtype=torch.float32, device=device)
-    hits = torch.zeros(nbins, dtype=torch.float32, device=device)
-    for c in range(n // per):
-        eng.filter_u16(u16_dev[c * per:(c + 1) * per])
-        fold_phase(eng.output_ptr(), per, period_s * bw, nbins, prof, hits, ndm=nd, stride=per, n0=c * per,
-                   stream=torch.cuda.current_stream())
-    torch.cuda.synchronize()
-    eng.close()
-    return snr_of_profile(prof.cpu().numpy().astype(np.float64), hits.cpu().numpy().astype(np.float64)[None, :])
+def power_for_snr(target_snr, n, duty, nbins=64):
+    """Period-averaged pulse power (noise power 1) whose folded profile of n
+    samples in nbins phase bins peaks target_snr noise sigmas above the
+    baseline: the peak-bin excess is the pulse peak power amp, the baseline
+    noise of a bin mean is 1/sqrt(n/nbins) (exponentially distributed power).
+    A fold of only part of the samples measures sqrt(part) times this."""
+    sigma = duty / 2.3548
+    gmean = sigma * math.sqrt(2 * math.pi) * math.erf(0.5 / (sigma * math.sqrt(2)))
+    return target_snr / math.sqrt(n / nbins) * gmean
+
+
+class Judge:
+    """Plain coherent method as judge: dedisperse at a set of trials of a fixed
+    fine DM grid (engine, FP32 fused form) and phase-fold every call's output;
+    engines are cached per trial range so repeated calls only launch kernels."""
+
+    def __init__(self, bw, f0, fine_dms, n_samples, fftpoint=16384, count=32, nbins=64, device="cuda:0"):
+        self.bw, self.f0, self.fine = bw, f0, np.asarray(fine_dms, np.float32)
+        self.n, self.fftpoint, self.count, self.nbins, self.device = n_samples, fftpoint, count, nbins, device
+        self.engines = {}
+        self.trials_run = 0
+        # Fold only output past the longest filter transient of the grid (the
+        # first K-1 blocks of an engine ramp up from zero history, which would
+        # modulate the folded baseline): the same samples for every trial, as
+        # the reference's fold discards Nd samples (msosm_psrdada.cpp:111-112).
+        top = Plan(bw, [float(self.fine.max())], f0, fftpoint=fftpoint, count=count, form=FORM_FUSED_F32)
+        self.skip = top.K_max * top.M
+        if self.skip >= n_samples:
+            raise ValueError("input shorter than the filter transient of the largest DM")
+
+    def _engine(self, lo, hi):
+        key = (lo, hi)
+        if key not in self.engines:
+            plan = Plan(self.bw, self.fine[lo:hi], self.f0, fftpoint=self.fftpoint, count=self.count,
+                        form=FORM_FUSED_F32, chirp=CHIRP_REF_F32W)
+            self.engines[key] = Engine(plan)
+        return self.engines[key]
+
+    def snr_range(self, u16_dev, period_s, lo, hi):
+        """Folded S/N of fine trials lo..hi-1."""
+        import torch
+        eng = self._engine(lo, hi)
+        eng.reset()
+        per = eng.plan.count * eng.plan.M
+        nd = hi - lo
+        prof = torch.zeros((nd, self.nbins), dtype=torch.float32, device=self.device)
+        hits = torch.zeros(self.nbins, dtype=torch.float32, device=self.device)
+        torch.cuda.synchronize()  # input and zeroed profiles ready before the engine's stream reads them
+        for c in range(self.n // per):
+            eng.filter_u16(u16_dev[c * per:(c + 1) * per])
+            o = max(0, self.skip - c * per)
+            if o < per:
+                fold_phase(eng.output_ptr() + 8 * o, per - o, period_s * self.bw, self.nbins, prof, hits, ndm=nd,
+                           stride=per, n0=c * per + o, stream=eng.stream)
+        eng.synchronize()
+        self.trials_run += nd
+        return snr_of_profile(prof.cpu().numpy().astype(np.float64), hits.cpu().numpy().astype(np.float64)[None, :])
+
+    def snr_trials(self, u16_dev, period_s, idx, chunk=128):
+        """Folded S/N of arbitrary fine trials idx (one engine per chunk of
+        consecutive trials of the sorted index set)."""
+        idx = np.asarray(idx)
+        out = np.empty(len(idx))
+        order = np.argsort(idx)
+        srt = idx[order]
+        i = 0
+        while i < len(srt):
+            j = i + 1
+            while j < len(srt) and srt[j] == srt[j - 1] + 1 and j - i < chunk:
+                j += 1
+            out[order[i:j]] = self.snr_range(u16_dev, period_s, int(srt[i]), int(srt[j - 1]) + 1)
+            i = j
+        return out
+
+    def snr_full(self, u16_dev, period_s, chunk=128):
+        out = np.empty(len(self.fine))
+        for lo in range(0, len(self.fine), chunk):
+            hi = min(len(self.fine), lo + chunk)
+            out[lo:hi] = self.snr_range(u16_dev, period_s, lo, hi)
+        return out
+
+    def close(self):
+        for e in self.engines.values():
+            e.close()
+        self.engines = {}
 
 
 def soft_target(n_out, idx, sigma=2.0):
@@ -220,44 +287,97 @@ def topk(logits, k):
     return np.argsort(-logits, axis=-1)[..., :k]
 
 
-def make_dataset(n_examples, n_samples, bw, f0, dm_max, fine_dms, frontend, period_range, duty_range, snr_range,
-                 seed, device, label_halfwidth=8, fftpoint=16384, count=None):
-    """Simulate pulsars, run the front end, label each with the plain coherent
-    method over fine trials within +-label_halfwidth of the true DM."""
-    import torch
+def draw_pulsars(n_examples, dm_max, period_range, duty_range, snr_range, seed):
+    """Seeded pulsar parameters: DM uniform, period log-uniform, duty cycle
+    uniform, folded S/N uniform, plus a simulation seed per example."""
     rng = np.random.Generator(np.random.Philox(seed))
-    X = torch.empty((n_examples, len(frontend.dms), frontend.Tout), dtype=torch.float32, device=device)
-    meta = []
-    count = count or max(1, n_samples // (fftpoint // 2) // 4)
-    for i in range(n_examples):
+    out = []
+    for _ in range(n_examples):
         dm = float(rng.uniform(0, dm_max))
         P = float(np.exp(rng.uniform(np.log(period_range[0]), np.log(period_range[1]))))
         duty = float(rng.uniform(*duty_range))
         snr = float(rng.uniform(*snr_range))
-        sim_seed = int(rng.integers(1, 2 ** 31))
-        u = simulate(n_samples, bw, f0, dm, P, duty, snr, sim_seed, device)
+        out.append(dict(dm=dm, period=P, duty=duty, snr=snr, seed=int(rng.integers(1, 2 ** 31))))
+    return out
+
+
+def make_dataset(pulsars, n_samples, bw, f0, frontend, judge, label_halfwidth=8, label_smooth=0, nbins=64,
+                 device="cuda:0", progress=None):
+    """Simulate each pulsar, run the front end, and label it with the plain
+    coherent method: the fine trial with the highest folded S/N within
+    +-label_halfwidth trials of the true DM, after a running mean over
+    +-label_smooth trials (the S/N of neighbouring trials differs mostly by
+    noise when the pulse is wide compared with their smearing difference)."""
+    import torch
+    X = torch.empty((len(pulsars), len(frontend.dms), frontend.Tout), dtype=torch.float32, device=device)
+    nf = len(judge.fine)
+    labels = []
+    for i, m in enumerate(pulsars):
+        u = simulate(n_samples, bw, f0, m["dm"], m["period"], m["duty"],
+                     power_for_snr(m["snr"], n_samples, m["duty"], nbins), m["seed"], device)
         frontend(u, X[i])
-        j0 = int(np.argmin(np.abs(fine_dms - dm)))
-        idx = np.arange(max(0, j0 - label_halfwidth), min(len(fine_dms), j0 + label_halfwidth + 1))
-        s = folded_snr(u, n_samples, bw, f0, fine_dms[idx], P, fftpoint=fftpoint, count=count, device=device)
-        meta.append(dict(dm=dm, period=P, duty=duty, snr=snr, label=int(idx[int(np.argmax(s))]),
-                         label_snr=float(s.max()), seed=sim_seed))
-    return X, meta
+        j0 = int(np.argmin(np.abs(judge.fine - m["dm"])))
+        # trial windows on a stride of 32 keep the number of cached engines small
+        width = min(nf, 2 * label_halfwidth + 32)
+        lo = min(max(0, (j0 - label_halfwidth) // 32 * 32), nf - width)
+        s = judge.snr_range(u, m["period"], lo, lo + width)
+        if label_smooth > 0:
+            ker = np.ones(2 * label_smooth + 1)
+            s = np.convolve(s, ker, "same") / np.convolve(np.ones_like(s), ker, "same")
+        a, b = max(j0 - label_halfwidth, lo), min(j0 + label_halfwidth + 1, lo + width)
+        labels.append(a + int(np.argmax(s[a - lo:b - lo])))
+        if progress and (i + 1) % progress == 0:
+            print(f"  simulated and labelled {i + 1}/{len(pulsars)}", flush=True)
+    return X, np.array(labels)
 
 
-def train(net, X, labels, epochs=10, batch=16, lr=2e-3, sigma=2.0, seed=0):
+def train(net, X, labels, epochs=10, batch=16, lr=2e-3, sigma=2.0, seed=0, cosine=False, progress=False):
+    """Adam on soft Gaussian targets (sigma in fine trials) around each
+    label; with cosine the learning rate decays to zero over the run."""
     import torch
     rng = np.random.default_rng(seed)
     n = X.shape[0]
     T = torch.from_numpy(np.stack([soft_target(net.n_out, l, sigma) for l in labels])).to(X.device)
     losses = []
+    total = epochs * (n // batch)
+    it = 0
     for ep in range(epochs):
         order = rng.permutation(n)
         tot = 0.0
+        steps = 0
         for s in range(0, n - batch + 1, batch):
             idx = torch.from_numpy(order[s:s + batch]).to(X.device)
             loss = net.loss_and_grad(X.index_select(0, idx).contiguous(), T.index_select(0, idx).contiguous())
-            net.adam_step(lr=lr)
+            net.adam_step(lr=lr * (0.5 * (1 + math.cos(math.pi * it / total)) if cosine else 1.0))
+            it += 1
             tot += float(loss.item())
-        losses.append(tot / max(1, n // batch))
+            steps += 1
+        losses.append(tot / max(1, steps))
+        if progress:
+            print(f"  epoch {ep + 1}/{epochs}: loss {losses[-1]:.4f}", flush=True)
     return losses
+
+
+class Proposer:
+    """Learned DM proposal in front of the plain coherent method: front end ->
+    network -> top-k fine trials -> coherent dedispersion + fold of those
+    trials only; if the best proposed S/N is below fallback_snr the full grid
+    is searched."""
+
+    def __init__(self, net, frontend, judge, k=16, fallback_snr=None):
+        self.net, self.frontend, self.judge, self.k, self.fallback_snr = net, frontend, judge, k, fallback_snr
+
+    def search(self, u16_dev, period_s):
+        import torch
+        x = torch.empty((1, len(self.frontend.dms), self.frontend.Tout), dtype=torch.float32,
+                        device=self.frontend.device)
+        self.frontend(u16_dev, x[0])
+        logits = self.net.forward(x).cpu().numpy()[0]
+        idx = topk(logits, self.k)
+        s = self.judge.snr_trials(u16_dev, period_s, idx)
+        best = int(np.argmax(s))
+        if self.fallback_snr is not None and s[best] < self.fallback_snr:
+            full = self.judge.snr_full(u16_dev, period_s)
+            j = int(np.argmax(full))
+            return j, float(full[j]), len(self.judge.fine), True
+        return int(idx[best]), float(s[best]), self.k, False

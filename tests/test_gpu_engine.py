@@ -1,7 +1,7 @@
 """GPU parity of the dedispersion engine (all execution forms) against the
 FP64 same-plan oracle.  Gates: FP32 forms <= 1e-5 relative L2, the
-FP16-storage form <= 1e-3 relative L2 (per DM, whole stream including the
-zero-filled transient)."""
+FP16-storage and matrix-core forms <= 1e-3 relative L2 (per DM, whole
+stream including the zero-filled transient)."""
 import numpy as np
 import pytest
 
@@ -12,8 +12,9 @@ pytestmark = pytest.mark.gpu
 
 FP32_TOL = 1e-5
 MIXED_TOL = 1e-3
-FORMS = [ms.FORM_PLAIN_F32, ms.FORM_FUSED_F32, ms.FORM_FUSED_F16]
-TOL = {ms.FORM_PLAIN_F32: FP32_TOL, ms.FORM_FUSED_F32: FP32_TOL, ms.FORM_FUSED_F16: MIXED_TOL}
+FORMS = [ms.FORM_PLAIN_F32, ms.FORM_FUSED_F32, ms.FORM_FUSED_F16, ms.FORM_HANKEL_F16, ms.FORM_HANKEL_BF16X3]
+TOL = {ms.FORM_PLAIN_F32: FP32_TOL, ms.FORM_FUSED_F32: FP32_TOL, ms.FORM_FUSED_F16: MIXED_TOL,
+       ms.FORM_HANKEL_F16: MIXED_TOL, ms.FORM_HANKEL_BF16X3: MIXED_TOL}
 
 
 def rel_l2(a, b):
