@@ -9,8 +9,12 @@ the whole input through all 256 trials (128 calls), input resident in HBM,
 output written per call to the engine's complex64 output buffer.
 
 The headline value is the FP16-storage fused form; the plain FP32 form (the
-reference's product-ring structure) is timed beside it for the ratio.  Both
-are checked against the FP64 same-plan oracle on a block range.
+reference's product-ring structure) is timed beside it for the ratio, and
+the matrix-core (per-bin Hankel GEMM) forms for comparison.  Every form is
+checked against the FP64 same-plan oracle on a block range.  Per-kernel
+rooflines: the dominant kernel of the headline form, the forward FFT, the
+plain form's inverse FFT, the GEMM kernel (useful and issued MFMA flops) and
+the phase fold over the output.
 
 Metric: DM-trial * Msamples / s = num_dms * input_samples / time / 1e6.
 """
@@ -28,6 +32,8 @@ sys.path.insert(0, ROOT)
 METRIC = "DM-trial·Msamples/s on 1 MI355X, mixed vs FP32 plain at ≤1e-3 rel err"
 UNIT = "DM-trial·Msamples/s"
 HBM_PEAK_GBS = 8000.0
+MFMA_F16_PEAK_TFLOPS = 2500.0
+FORM_NAMES = ("plain_f32", "mixed_f16", "hankel_f16", "hankel_bf16x3")
 
 
 def parse():
@@ -39,7 +45,13 @@ def parse():
     ap.add_argument("--num-dms", type=int, default=256)
     ap.add_argument("--count", type=int, default=64)
     ap.add_argument("--fftpoint", type=int, default=16384)
+    ap.add_argument("--forms", default=",".join(FORM_NAMES),
+                    help="comma list of forms to time (mixed_f16 is the headline and always runs)")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP32 comparison")
+    ap.add_argument("--fold-bins", type=int, default=512)
+    ap.add_argument("--pmc-workload", action="store_true",
+                    help="short run for hardware-counter passes: a few calls of the headline form and a 1 GiB "
+                         "device copy (the counter calibration), no timing")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-check", action="store_true", help="skip the FP64 spot check")
     ap.add_argument("--cpu-log2-samples", type=int, default=25)
@@ -66,6 +78,9 @@ def main():
     L = 1 << args.log2_samples
     bw, f0 = 64e6, 1368e6
     dms = np.linspace(0.0, 100.0, args.num_dms).astype(np.float32)
+    if args.pmc_workload:
+        pmc_workload(ms, torch, dev, bw, f0, dms, args)
+        return
     t0 = time.time()
     u = simulate.pulsar_u16(L, bw, f0, 57.3, 0.008, duty=0.05, snr=0.1, seed=0x5EED0003)
     log(f"input: {L} samples generated in {time.time() - t0:.1f}s")
@@ -73,10 +88,17 @@ def main():
     stream = torch.cuda.current_stream()
     results = {}
     checks = {}
-    forms = [("mixed_f16", ms.FORM_FUSED_F16)]
-    if not args.no_plain:
-        forms.insert(0, ("plain_f32", ms.FORM_PLAIN_F32))
+    form_ids = {"plain_f32": ms.FORM_PLAIN_F32, "mixed_f16": ms.FORM_FUSED_F16, "hankel_f16": ms.FORM_HANKEL_F16,
+                "hankel_bf16x3": ms.FORM_HANKEL_BF16X3}
+    wanted = [f for f in args.forms.split(",") if f]
+    if "mixed_f16" not in wanted:
+        wanted.append("mixed_f16")
+    if args.no_plain and "plain_f32" in wanted:
+        wanted.remove("plain_f32")
+    forms = [(f, form_ids[f]) for f in FORM_NAMES if f in wanted]
     plan_info = None
+    fold = None
+    hankel_flops = None
     for name, form in forms:
         plan = ms.Plan(bw, dms, f0, fftpoint=args.fftpoint, count=args.count, form=form)
         n = plan.count * plan.M
@@ -111,6 +133,12 @@ def main():
             for k, v in timing.items() if v[1]})
         log(f"{name}: {ms_step:.2f} ms/step -> {value:.1f} {UNIT}; stages "
             + ", ".join(f"{k}={v['ms_per_launch']:.3f}ms" for k, v in results[name]["stages"].items()))
+        if name == "mixed_f16":
+            fold = fold_bench(ms, torch, eng, plan, stream, 0.008 * bw, args.fold_bins)
+            log(f"phase fold ({args.fold_bins} bins): {fold['ms_per_launch']:.3f} ms per launch, "
+                f"{fold['achieved']:.0f} GB/s")
+        if name.startswith("hankel") and hankel_flops is None:
+            hankel_flops = hankel_flop_count(plan)
         if not args.no_check:
             checks[name] = spot_check(ms, eng, plan, d_in, u, bw, f0, dms, n)
             log(f"{name}: max rel L2 vs FP64 on spot check = {checks[name]['max_rel_l2']:.3e}")
@@ -124,8 +152,11 @@ def main():
     alg_bytes = units * plan_info["M"] * (8 + 8)  # gather 2 FP16 bins + write 1 complex64 per DM-sample
     achieved = alg_bytes / (st["ms_per_launch"] / 1e3) / 1e9
     roofline = dict(bound="hbm", kernel="k_inv_fused<14,fp16>", achieved=round(achieved, 1), peak=HBM_PEAK_GBS,
-                    unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                    unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 4),
+                    traffic=measured_traffic(args, len(dms)),
                     alg_bytes_per_launch=alg_bytes, ms_per_launch=st["ms_per_launch"])
+    kernels = kernel_rooflines(results, plan_info, len(dms), fold, hankel_flops)
+    kernels["k_inv_fused<14,fp16>"] = {k: roofline[k] for k in ("achieved", "unit", "peak", "frac", "ms_per_launch")}
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(bw, f0, dms, args.fftpoint // 2, u, args.cpu_log2_samples)
@@ -156,9 +187,126 @@ def main():
                   for k, v in results.items()},
         "ratio_mixed_vs_plain_f32": (round(mixed["value"] / results["plain_f32"]["value"], 3)
                                      if "plain_f32" in results else None),
+        "kernels": kernels,
         "rel_l2_vs_fp64": {k: v for k, v in checks.items()},
     }
     print(json.dumps(line), flush=True)
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_multidm_2e14_256dm.md")
+
+
+def measured_traffic(args, ndm):
+    """HBM bytes per launch of the fused inverse from the committed counter
+    summary (FETCH_SIZE and WRITE_SIZE passes scaled by a 1 GiB copy); only
+    for the workload it was measured on."""
+    if (ndm, args.count, args.fftpoint) != (256, 64, 16384) or not os.path.exists(TRAFFIC_FILE):
+        return None
+    for line in open(TRAFFIC_FILE):
+        if line.startswith("| `k_inv_fused_nb"):
+            total = line.split("|")[6].strip().strip("*").split()[0]
+            return int(float(total) * 1e9)
+    return None
+
+
+def pmc_workload(ms, torch, dev, bw, f0, dms, args, calls=6):
+    """Dispatches for counter passes (FETCH_SIZE, WRITE_SIZE): `calls` calls
+    of the headline form on random offset-binary input, then two copies of
+    1 GiB between device buffers -- a kernel whose HBM bytes are known
+    (read 1 GiB, write 1 GiB), giving the counters' scale on this device."""
+    plan = ms.Plan(bw, dms, f0, fftpoint=args.fftpoint, count=args.count, form=ms.FORM_FUSED_F16)
+    n = plan.count * plan.M
+    g = torch.Generator(device=dev).manual_seed(7)
+    lo = torch.randint(1, 65536, (calls * n,), generator=g, device=dev, dtype=torch.int32)
+    hi = torch.randint(1, 65536, (calls * n,), generator=g, device=dev, dtype=torch.int32)
+    d_in = lo | (hi << 16)
+    eng = ms.Engine(plan, stream=torch.cuda.current_stream())
+    for c in range(calls):
+        eng.filter_u16(d_in[c * n:(c + 1) * n])
+    torch.cuda.synchronize()
+    eng.close()
+    src = torch.ones(1 << 28, dtype=torch.float32, device=dev)
+    dst = torch.empty_like(src)
+    for _ in range(2):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"pmc_workload": {"calls": calls, "units_per_call": len(dms) * plan.count,
+                                       "copy_bytes": src.numel() * 4}}), flush=True)
+
+
+def fold_bench(ms, torch, eng, plan, stream, period_samples, nbins, launches=20):
+    """Phase fold of the engine's last output (all trials, count*M samples
+    each) into nbins bins, timed with HIP events on the engine stream;
+    algorithmic bytes = 8 B (complex64) read per DM-sample."""
+    n = plan.count * plan.M
+    prof = torch.zeros((plan.num_dms, nbins), dtype=torch.float32, device="cuda:0")
+    hits = torch.zeros(nbins, dtype=torch.float32, device="cuda:0")
+    ptr = eng.output_ptr()
+    ms.fold_phase(ptr, n, period_samples, nbins, prof, hits, ndm=plan.num_dms, stride=n, stream=stream)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for i in range(launches):
+        ms.fold_phase(ptr, n, period_samples, nbins, prof, hits, ndm=plan.num_dms, stride=n, n0=i * n, stream=stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    t = e0.elapsed_time(e1) / launches
+    alg = plan.num_dms * n * 8
+    return dict(ms_per_launch=t, alg_bytes_per_launch=alg, achieved=alg / (t / 1e3) / 1e9)
+
+
+def hankel_flop_count(plan):
+    """Useful and issued flops of one GEMM launch.  Useful: one complex MAC
+    (8 real flops) per (trial, bin, block).  Issued: per (bin, 16-trial tile,
+    16-block tile) the 32-deep MFMA steps that cover the tile's delay band,
+    2 MFMAs (real, imaginary) of 16x16x32 each; the dense recast would issue
+    8 K_max per (trial, bin, block)."""
+    N, count, ndm = plan.fftpoint, plan.count, plan.num_dms
+    d = np.stack([plan.delay_points(i) for i in range(ndm)])  # [ndm][N]
+    steps = 0
+    for t0 in range(0, ndm, 16):
+        blk = d[t0:t0 + 16]
+        steps += int(((blk.max(axis=0) - blk.min(axis=0)) // 16 + 1).sum())
+    gtiles = (count + 15) // 16
+    issued_per_term = steps * gtiles * 2 * (16 * 16 * 32 * 2)
+    useful = 8 * ndm * N * count
+    return dict(useful=useful, issued_per_term=issued_per_term, dense=8 * plan.K_max * ndm * N * count)
+
+
+def kernel_rooflines(results, plan_info, ndm, fold, hankel_flops):
+    M, count = plan_info["M"], plan_info["count"]
+    N = 2 * M
+    out = {}
+
+    def put(name, alg, ms_launch, unit="GB/s", peak=HBM_PEAK_GBS, **extra):
+        if not ms_launch:
+            return
+        a = alg / (ms_launch / 1e3) / (1e9 if unit == "GB/s" else 1e12)
+        out[name] = dict(achieved=round(a, 1), unit=unit, peak=peak, frac=round(a / peak, 4),
+                         ms_per_launch=round(ms_launch, 4), **extra)
+
+    st = results["mixed_f16"]["stages"]
+    # forward FFT: count windows of N points, uint16 in (4 B) and FP16 out (4 B)
+    put("k_fwd_u16<14,fp16>", count * N * 8, st["fft_fwd"]["ms_per_launch"])
+    if "plain_f32" in results:
+        sp = results["plain_f32"]["stages"]
+        put("k_fft_c2c<14,inverse> (plain)", ndm * count * N * 16, sp["fft_inv"]["ms_per_launch"])
+        # per (trial, bin): read H, write the ring slot (8 B each); the spectrum is shared by all trials
+        put("k_multiply_delay (plain)", ndm * count * N * 16, sp["multiply"]["ms_per_launch"])
+    for name, mode, terms in (("hankel_f16", "fp16", 1), ("hankel_bf16x3", "bf16x3", 3)):
+        if name not in results or hankel_flops is None:
+            continue
+        sh = results[name]["stages"]
+        t = sh["multiply"]["ms_per_launch"]
+        issued = hankel_flops["issued_per_term"] * terms
+        put(f"k_hankel<{mode}> (useful flops)", hankel_flops["useful"], t, unit="TFLOP/s",
+            peak=MFMA_F16_PEAK_TFLOPS, issued_tflops=round(issued / (t / 1e3) / 1e12, 1),
+            issued_over_useful=round(issued / hankel_flops["useful"], 2),
+            dense_over_useful=round(hankel_flops["dense"] / hankel_flops["useful"], 1))
+        ybytes = 4 if mode == "fp16" else 8
+        put(f"k_inv_dense<14,{mode}>", ndm * count * (N * ybytes + M * 8), sh["fft_inv"]["ms_per_launch"])
+    if fold:
+        put("k_fold_phase", fold["alg_bytes_per_launch"], fold["ms_per_launch"])
+    return out
 
 
 def spot_check(ms, eng, plan, d_in, u, bw, f0, dms, n):

@@ -76,9 +76,24 @@ __global__ void k_fold_bins(float *__restrict__ folded, const float *__restrict_
 // workgroup builds a histogram in LDS (float atomics) and adds it to the
 // global profile once.  value = sqrt(|a|^2+|b|^2) (the reference's
 // intensity, gpu_kernel.cu:352) or |a|^2+|b|^2 (power).
-__global__ void k_fold_phase(const float2 *__restrict__ a, const float2 *__restrict__ b, long long n, long long stride,
-                             long long n0, double inv_period, int nbins, int power, float *__restrict__ prof,
-                             float *__restrict__ hits)
+//
+// The loop is wave-uniform (64 consecutive samples per wave and step, idle
+// lanes contribute nothing).  When a period spans many samples per bin, all
+// lanes of a wave usually share one bin: the wave then sums its values with
+// lane shuffles and one lane issues a single LDS atomic instead of 64
+// conflicting ones.
+__device__ __forceinline__ float wave_sum(float x)
+{
+#pragma unroll
+    for (int m = 32; m > 0; m >>= 1)
+        x += __shfl_xor(x, m);
+    return x;
+}
+
+__global__ __launch_bounds__(256) void k_fold_phase(const float2 *__restrict__ a, const float2 *__restrict__ b,
+                                                    long long n, long long stride, long long n0, double inv_period,
+                                                    int nbins, int power, float *__restrict__ prof,
+                                                    float *__restrict__ hits)
 {
     extern __shared__ float hist[]; // nbins values (+ nbins hit counts for trial 0)
     const int dm = blockIdx.y;
@@ -89,23 +104,42 @@ __global__ void k_fold_phase(const float2 *__restrict__ a, const float2 *__restr
     __syncthreads();
     const float2 *ad = a + (long long)dm * stride;
     const float2 *bd = b ? b + (long long)dm * stride : nullptr;
-    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
-         i += (long long)gridDim.x * blockDim.x) {
-        float2 v = ad[i];
-        float p = v.x * v.x + v.y * v.y;
-        if (bd) {
-            float2 w = bd[i];
-            p += w.x * w.x + w.y * w.y;
+    const int lane = threadIdx.x & 63;
+    const long long wave0 = (blockIdx.x * (long long)blockDim.x + threadIdx.x) - lane;
+    const long long step = (long long)gridDim.x * blockDim.x;
+    for (long long base = wave0; base < n; base += step) { // wave-uniform bound
+        const long long i = base + lane;
+        const bool valid = i < n;
+        float p = 0.f;
+        int bin = 0;
+        if (valid) {
+            float2 v = ad[i];
+            p = v.x * v.x + v.y * v.y;
+            if (bd) {
+                float2 w = bd[i];
+                p += w.x * w.x + w.y * w.y;
+            }
+            if (!power)
+                p = sqrtf(p);
+            double ph = (double)(n0 + i) * inv_period;
+            ph -= floor(ph);
+            bin = (int)(ph * (double)nbins);
+            bin = bin < nbins ? bin : nbins - 1;
         }
-        if (!power)
-            p = sqrtf(p);
-        double ph = (double)(n0 + i) * inv_period;
-        ph -= floor(ph);
-        int bin = (int)(ph * (double)nbins);
-        bin = bin < nbins ? bin : nbins - 1;
-        atomicAdd(&hist[bin], p);
-        if (count_hits)
-            atomicAdd(&hist[nbins + bin], 1.0f);
+        const int b0 = __shfl(bin, 0);
+        if (__all(!valid || bin == b0)) {
+            const float s = wave_sum(p);
+            const float c = wave_sum(valid ? 1.0f : 0.0f);
+            if (lane == 0) {
+                atomicAdd(&hist[b0], s);
+                if (count_hits)
+                    atomicAdd(&hist[nbins + b0], c);
+            }
+        } else if (valid) {
+            atomicAdd(&hist[bin], p);
+            if (count_hits)
+                atomicAdd(&hist[nbins + bin], 1.0f);
+        }
     }
     __syncthreads();
     for (int i = threadIdx.x; i < nbins; i += blockDim.x) {

@@ -77,12 +77,15 @@ def test_intensity_then_fold_end_to_end(device):
     fold.close()
 
 
-def test_phase_fold_lds_histogram(device):
+@pytest.mark.parametrize("P", [1234.567, 987654.321])
+def test_phase_fold_lds_histogram(device, P):
     # FP64 phase per sample, LDS histogram per workgroup: bin assignment
-    # (hit counts) exact, sums to float tolerance, for 3 trials at once
+    # (hit counts) exact, sums to float tolerance, for 3 trials at once; the
+    # long period puts whole waves into one bin (wave-sum path), the short
+    # one splits most waves over several bins
     import torch
     rng = np.random.default_rng(9)
-    ndm, n, stride, nbins, P, n0 = 3, 100000, 100352, 64, 1234.567, 5000
+    ndm, n, stride, nbins, n0 = 3, 100000, 100352, 64, 5000
     a = (rng.standard_normal((ndm, stride)) + 1j * rng.standard_normal((ndm, stride))).astype(np.complex64)
     b = (rng.standard_normal((ndm, stride)) + 1j * rng.standard_normal((ndm, stride))).astype(np.complex64)
     ad = torch.from_numpy(a).to(device)
@@ -100,4 +103,4 @@ def test_phase_fold_lds_histogram(device):
         if not power:
             v = np.sqrt(v)
         exp = np.stack([np.bincount(bins, weights=v[d], minlength=nbins) for d in range(ndm)])
-        assert np.max(np.abs(prof.cpu().numpy() - exp) / exp) < 1e-5
+        assert np.max(np.abs(prof.cpu().numpy() - exp) / np.maximum(exp, 1e-30)) < 1e-5
