@@ -77,10 +77,13 @@ __global__ void k_fold_bins(float *__restrict__ folded, const float *__restrict_
 // global profile once.  value = sqrt(|a|^2+|b|^2) (the reference's
 // intensity, gpu_kernel.cu:352) or |a|^2+|b|^2 (power).
 //
-// The loop is wave-uniform (64 consecutive samples per wave and step, idle
-// lanes contribute nothing).  When a period spans many samples per bin, all
-// lanes of a wave usually share one bin: the wave then sums its values with
-// lane shuffles and one lane issues a single LDS atomic instead of 64
+// Each workgroup folds one contiguous chunk of its trial's samples, so it
+// touches only the bins its chunk spans and adds only those (non-zero) bins
+// to the global profile.  The loop is wave-uniform: every wave takes 4
+// groups of 64 consecutive samples per step (4 loads in flight), idle lanes
+// contribute nothing.  When a period spans many samples per bin, all lanes
+// of a group usually share one bin: the wave then sums its values with lane
+// shuffles and one lane issues a single LDS atomic instead of 64
 // conflicting ones.
 __device__ __forceinline__ float wave_sum(float x)
 {
@@ -90,10 +93,14 @@ __device__ __forceinline__ float wave_sum(float x)
     return x;
 }
 
-__global__ __launch_bounds__(256) void k_fold_phase(const float2 *__restrict__ a, const float2 *__restrict__ b,
-                                                    long long n, long long stride, long long n0, double inv_period,
-                                                    int nbins, int power, float *__restrict__ prof,
-                                                    float *__restrict__ hits)
+constexpr int kFoldThreads = 256;
+constexpr int kFoldUnroll = 4;
+
+__global__ __launch_bounds__(kFoldThreads) void k_fold_phase(const float2 *__restrict__ a,
+                                                             const float2 *__restrict__ b, long long n,
+                                                             long long stride, long long chunk, long long n0,
+                                                             double inv_period, int nbins, int power,
+                                                             float *__restrict__ prof, float *__restrict__ hits)
 {
     extern __shared__ float hist[]; // nbins values (+ nbins hit counts for trial 0)
     const int dm = blockIdx.y;
@@ -104,48 +111,65 @@ __global__ __launch_bounds__(256) void k_fold_phase(const float2 *__restrict__ a
     __syncthreads();
     const float2 *ad = a + (long long)dm * stride;
     const float2 *bd = b ? b + (long long)dm * stride : nullptr;
-    const int lane = threadIdx.x & 63;
-    const long long wave0 = (blockIdx.x * (long long)blockDim.x + threadIdx.x) - lane;
-    const long long step = (long long)gridDim.x * blockDim.x;
-    for (long long base = wave0; base < n; base += step) { // wave-uniform bound
-        const long long i = base + lane;
-        const bool valid = i < n;
-        float p = 0.f;
-        int bin = 0;
-        if (valid) {
-            float2 v = ad[i];
-            p = v.x * v.x + v.y * v.y;
-            if (bd) {
-                float2 w = bd[i];
-                p += w.x * w.x + w.y * w.y;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const long long c0 = (long long)blockIdx.x * chunk;
+    const long long c1 = c0 + chunk < n ? c0 + chunk : n;
+    constexpr int waves = kFoldThreads / 64;
+    for (long long base = c0 + (long long)wave * 64 * kFoldUnroll; base < c1; base += waves * 64 * kFoldUnroll) {
+        float p[kFoldUnroll];
+#pragma unroll
+        for (int u = 0; u < kFoldUnroll; u++) {
+            const long long i = base + u * 64 + lane;
+            p[u] = 0.f;
+            if (i < c1) {
+                const float2 v = ad[i];
+                p[u] = v.x * v.x + v.y * v.y;
+                if (bd) {
+                    const float2 w = bd[i];
+                    p[u] += w.x * w.x + w.y * w.y;
+                }
             }
-            if (!power)
-                p = sqrtf(p);
-            double ph = (double)(n0 + i) * inv_period;
-            ph -= floor(ph);
-            bin = (int)(ph * (double)nbins);
-            bin = bin < nbins ? bin : nbins - 1;
         }
-        const int b0 = __shfl(bin, 0);
-        if (__all(!valid || bin == b0)) {
-            const float s = wave_sum(p);
-            const float c = wave_sum(valid ? 1.0f : 0.0f);
-            if (lane == 0) {
-                atomicAdd(&hist[b0], s);
-                if (count_hits)
-                    atomicAdd(&hist[nbins + b0], c);
+#pragma unroll
+        for (int u = 0; u < kFoldUnroll; u++) {
+            const long long i = base + u * 64 + lane;
+            const bool valid = i < c1;
+            float v = p[u];
+            if (!power)
+                v = sqrtf(v);
+            int bin = 0;
+            if (valid) {
+                double ph = (double)(n0 + i) * inv_period;
+                ph -= floor(ph);
+                bin = (int)(ph * (double)nbins);
+                bin = bin < nbins ? bin : nbins - 1;
             }
-        } else if (valid) {
-            atomicAdd(&hist[bin], p);
-            if (count_hits)
-                atomicAdd(&hist[nbins + bin], 1.0f);
+            const int b0 = __shfl(bin, 0);
+            if (__all(!valid || bin == b0)) {
+                const float sum = wave_sum(v);
+                const float cnt = count_hits ? wave_sum(valid ? 1.0f : 0.0f) : 0.0f;
+                if (lane == 0 && base + u * 64 < c1) {
+                    atomicAdd(&hist[b0], sum);
+                    if (count_hits)
+                        atomicAdd(&hist[nbins + b0], cnt);
+                }
+            } else if (valid) {
+                atomicAdd(&hist[bin], v);
+                if (count_hits)
+                    atomicAdd(&hist[nbins + bin], 1.0f);
+            }
         }
     }
     __syncthreads();
     for (int i = threadIdx.x; i < nbins; i += blockDim.x) {
-        atomicAdd(&prof[(long long)dm * nbins + i], hist[i]);
-        if (count_hits)
-            atomicAdd(&hits[i], hist[nbins + i]);
+        const float h = hist[i];
+        if (h != 0.0f)
+            atomicAdd(&prof[(long long)dm * nbins + i], h);
+        if (count_hits) {
+            const float c = hist[nbins + i];
+            if (c != 0.0f)
+                atomicAdd(&hits[i], c);
+        }
     }
 }
 
@@ -503,15 +527,17 @@ int msosm_fold_phase(const void *a, const void *b, size_t n, int ndm, size_t str
                                    "stride >= n)");
     if (ndm > 65535)
         return ferr(MSOSM_ERR_ARG, "fold_phase: at most 65535 DM trials per launch");
-    const int threads = 256;
-    long long gx = (long long)((n + threads * 16 - 1) / (threads * 16));
-    if (gx > 1024)
-        gx = 1024;
-    if (gx < 1)
-        gx = 1;
+    // contiguous chunks of at least 16384 samples (a multiple of one step of
+    // the 4 waves), at most 4096 chunks per trial
+    const long long step = (long long)kFoldThreads * kFoldUnroll;
+    long long chunk = 16384;
+    while ((long long)(n + chunk - 1) / chunk > 4096)
+        chunk *= 2;
+    chunk = (chunk + step - 1) / step * step;
+    const long long gx = ((long long)n + chunk - 1) / chunk;
     size_t lds = sizeof(float) * (size_t)nbins * (hits ? 2 : 1);
-    hipLaunchKernelGGL(k_fold_phase, dim3((unsigned)gx, ndm), dim3(threads), lds, (hipStream_t)stream,
-                       (const float2 *)a, (const float2 *)b, (long long)n, (long long)stride, n0,
+    hipLaunchKernelGGL(k_fold_phase, dim3((unsigned)gx, ndm), dim3(kFoldThreads), lds, (hipStream_t)stream,
+                       (const float2 *)a, (const float2 *)b, (long long)n, (long long)stride, chunk, n0,
                        1.0 / period_samples, nbins, power, prof, hits);
     FHIP(hipGetLastError());
     return MSOSM_OK;

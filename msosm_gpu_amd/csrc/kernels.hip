@@ -743,16 +743,18 @@ constexpr int kInvBlocksPerWg = 8;
 // Inverse-kernel variant for the sizes with one transform per workgroup:
 // 0 = one (DM, block) per workgroup, 1 = kInvBlocksPerWg blocks per
 // workgroup (default), 2 = as 1 with the next block's gather prefetched into
-// registers, 3 = as 2 with 16 points per thread for 2^14 (1024 threads).
+// registers, 3 = as 2 with 16 points per thread for 2^14 (1024 threads),
+// 4 = as 1 with 16 points per thread for 2^14.
 // Measured on MI355X for 256 trials x 64 blocks of 2^14 points (FP16 ring):
-// 2.13 / 0.66 / 1.75 / 1.75 ms per launch -- the prefetch costs register
-// spills that outweigh the overlap.  MSOSM_INV_KERNEL overrides the choice.
+// 2.13 / 0.66 / 1.75 / 1.75 ms per launch for 0-3 -- the prefetch costs
+// register spills that outweigh the overlap.  MSOSM_INV_KERNEL overrides
+// the choice.
 static int inv_variant()
 {
     static int v = [] {
         const char *e = getenv("MSOSM_INV_KERNEL");
         int x = e ? atoi(e) : 1;
-        return (x >= 0 && x <= 3) ? x : 1;
+        return (x >= 0 && x <= 4) ? x : 1;
     }();
     return v;
 }
@@ -777,11 +779,15 @@ static hipError_t inv_fused(const void *ring, const void *H, const uint16_t *del
     const int var = inv_variant();
     if (fft_tpb(LOG2N) == 1 && var > 0) {
         if constexpr (fft_tpb(LOG2N) == 1) {
-            if (LOG2N == 14 && var == 3) {
-                if (f16)
+            if (LOG2N == 14 && var >= 3) {
+                if (f16 && var == 3)
                     launch_nb<LOG2N, 16, true, true>(ring, H, dpk, out, count, ndm, g0, rx, s);
-                else
+                else if (f16)
+                    launch_nb<LOG2N, 16, true, false>(ring, H, dpk, out, count, ndm, g0, rx, s);
+                else if (var == 3)
                     launch_nb<LOG2N, 16, false, true>(ring, H, dpk, out, count, ndm, g0, rx, s);
+                else
+                    launch_nb<LOG2N, 16, false, false>(ring, H, dpk, out, count, ndm, g0, rx, s);
             } else if (f16) {
                 if (var >= 2)
                     launch_nb<LOG2N, E, true, true>(ring, H, dpk, out, count, ndm, g0, rx, s);
@@ -836,7 +842,7 @@ hipError_t launch_fwd_u16(const uint32_t *cur, const uint32_t *tail, void *ring,
 
 int fused_delay_threads(int log2n)
 {
-    if (log2n == 14 && inv_variant() == 3)
+    if (log2n == 14 && inv_variant() >= 3)
         return (1 << log2n) / 16;
     return (1 << log2n) / fft_e(log2n);
 }
