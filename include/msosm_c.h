@@ -164,6 +164,20 @@ int msosm_k_discard_batch(const float *a, float *b, int M, int count, int num_dm
  * forward, +1 inverse (unnormalised).  in and out must not overlap. */
 int msosm_k_fft_c2c(const float *in, float *out, size_t n, size_t batch, size_t idist, size_t odist,
                     int direction, void *stream);
+/* The multiply-delay of ComplexMultiplyDelay_concurrent_batch (:23,
+ * gpu_kernel.cu:160-197) over a shared spectra ring, as per-bin complex GEMMs
+ * on the matrix cores: Y[dm][b][n] = H[dm][n] * ring[(g0 + b - d[dm][n]) mod rx][n]
+ * (zero when g0 + b < d), for b < count, n < 2^log2n.  form: one of
+ * MSOSM_FORM_HANKEL_*.  HANKEL_F16: ring and H are half2 (X/N, H/M), Y half2
+ * holding the product / 2; BF16 forms: ring, H, Y complex64.  kmax: 1 + the
+ * largest delay in the table. */
+int msosm_k_hankel_delay_gemm(const void *ring, const void *H, const uint16_t *delay, void *Y, int form,
+                              int log2n, int count, int num_dms, long long g0, int rx, int kmax, void *stream);
+/* Inverse FFT of rows Y[u][2^log2n] (half2 if y_f16, else complex64), keeping
+ * the last 2^(log2n-1) points times scale: out[u][2^(log2n-1)] (the inverse
+ * transform and DiscardSamples_concurrent_batch, :24, fused). */
+int msosm_k_inverse_discard(const void *Y, float *out, int y_f16, int log2n, size_t units, float scale,
+                            void *stream);
 
 /* Fold (folding/include/fold_gpu.h:10-41).  The fold runs on the GPU; bin
  * assignment follows Fold_GPU::fold_data exactly (fold_gpu.cpp:48-173). */

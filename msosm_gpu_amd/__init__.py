@@ -91,6 +91,8 @@ SIGNATURES = {
     "msosm_k_multiply_delay_batch": (_i, [_vp, _vp, _i, _vp, _vp, _i, _i, _i, _i, _vp]),
     "msosm_k_discard_batch": (_i, [_vp, _vp, _i, _i, _i, _vp]),
     "msosm_k_fft_c2c": (_i, [_vp, _vp, _sz, _sz, _sz, _sz, _i, _vp]),
+    "msosm_k_hankel_delay_gemm": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _i, _ll, _i, _i, _vp]),
+    "msosm_k_inverse_discard": (_i, [_vp, _vp, _i, _i, _sz, _f, _vp]),
     "msosm_fold_create": (_i, [_f, _f, _ul, _ul, _vp, ctypes.POINTER(_vp)]),
     "msosm_fold_destroy": (_i, [_vp]),
     "msosm_fold_discard_samples": (_i, [_vp, _ul]),
@@ -390,6 +392,18 @@ def multiply_delay_batch(a, b, N, ring, delay, delay_block_size, index, count, n
 
 def discard_batch(a, b, M, count, num_dms=1, stream=None):
     check(lib().msosm_k_discard_batch(_ptr(a), _ptr(b), M, count, num_dms, _stream_handle(stream)))
+
+
+def hankel_delay_gemm(ring, H, delay, Y, form, log2n, count, num_dms, g0, rx, kmax, stream=None):
+    """Per-bin complex GEMMs of the delay line on the matrix cores (device
+    tensors; see msosm_k_hankel_delay_gemm)."""
+    check(lib().msosm_k_hankel_delay_gemm(_ptr(ring), _ptr(H), _ptr(delay), _ptr(Y), form, log2n, count, num_dms,
+                                          g0, rx, kmax, _stream_handle(stream)))
+
+
+def inverse_discard(Y, out, y_f16, log2n, units, scale=1.0, stream=None):
+    check(lib().msosm_k_inverse_discard(_ptr(Y), _ptr(out), 1 if y_f16 else 0, log2n, units, scale,
+                                        _stream_handle(stream)))
 
 
 def fft_c2c(inp, out, n, batch, idist=None, odist=None, direction=-1, stream=None):

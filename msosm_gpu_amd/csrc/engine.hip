@@ -792,6 +792,31 @@ int msosm_k_discard_batch(const float *a, float *b, int M, int count, int ndm, v
     return MSOSM_OK;
 }
 
+int msosm_k_hankel_delay_gemm(const void *ring, const void *H, const uint16_t *delay, void *Y, int form, int log2n,
+                              int count, int num_dms, long long g0, int rx, int kmax, void *stream)
+{
+    if (!ring || !H || !delay || !Y || !form_hankel(form) || log2n < 5 || log2n > 14 || count < 1 ||
+        num_dms < 1 || rx < count || kmax < 1 || kmax > rx - count + 1 || g0 < 0)
+        return set_error(MSOSM_ERR_ARG, "hankel_delay_gemm: bad argument (Hankel form, 5 <= log2n <= 14, "
+                                        "rx >= count, 1 <= kmax <= rx - count + 1)");
+    const int mode = hankel_mode_of(form);
+    if (kmax > msosm::hankel_max_kmax(mode))
+        return set_error(MSOSM_ERR_UNSUPPORTED, "hankel_delay_gemm: kmax %d above %d", kmax,
+                         msosm::hankel_max_kmax(mode));
+    HIP_TRY(msosm::launch_hankel(ring, H, delay, Y, mode, log2n, count, num_dms, g0, rx, kmax,
+                                 (hipStream_t)stream));
+    return MSOSM_OK;
+}
+
+int msosm_k_inverse_discard(const void *Y, float *out, int y_f16, int log2n, size_t units, float scale, void *stream)
+{
+    if (!Y || !out || log2n < 2 || log2n > 14 || units < 1)
+        return set_error(MSOSM_ERR_ARG, "inverse_discard: bad argument (2 <= log2n <= 14, units >= 1)");
+    HIP_TRY(msosm::launch_inv_dense(Y, (float2 *)out, y_f16 != 0, log2n, (long long)units, scale,
+                                    (hipStream_t)stream));
+    return MSOSM_OK;
+}
+
 int msosm_k_fft_c2c(const float *in, float *out, size_t n, size_t batch, size_t idist, size_t odist, int direction,
                     void *stream)
 {

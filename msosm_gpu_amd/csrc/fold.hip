@@ -10,6 +10,8 @@
 // the sequential fold: folded_data_raw is reproduced bit for bit.
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
+
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -79,8 +81,8 @@ __global__ void k_fold_bins(float *__restrict__ folded, const float *__restrict_
 //
 // Each workgroup folds one contiguous chunk of its trial's samples, so it
 // touches only the bins its chunk spans and adds only those (non-zero) bins
-// to the global profile.  The loop is wave-uniform: every wave takes 4
-// groups of 64 consecutive samples per step (4 loads in flight), idle lanes
+// to the global profile.  The loop is wave-uniform: every wave takes 8
+// groups of 64 consecutive samples per step (8 loads in flight), idle lanes
 // contribute nothing.  When a period spans many samples per bin, all lanes
 // of a group usually share one bin: the wave then sums its values with lane
 // shuffles and one lane issues a single LDS atomic instead of 64
@@ -94,7 +96,7 @@ __device__ __forceinline__ float wave_sum(float x)
 }
 
 constexpr int kFoldThreads = 256;
-constexpr int kFoldUnroll = 4;
+constexpr int kFoldUnroll = 8;
 
 __global__ __launch_bounds__(kFoldThreads) void k_fold_phase(const float2 *__restrict__ a,
                                                              const float2 *__restrict__ b, long long n,
@@ -528,9 +530,15 @@ int msosm_fold_phase(const void *a, const void *b, size_t n, int ndm, size_t str
     if (ndm > 65535)
         return ferr(MSOSM_ERR_ARG, "fold_phase: at most 65535 DM trials per launch");
     // contiguous chunks of at least 16384 samples (a multiple of one step of
-    // the 4 waves), at most 4096 chunks per trial
+    // the 4 waves), at most 4096 chunks per trial; MSOSM_FOLD_CHUNK overrides
+    // the minimum (tuning runs)
+    static const long long min_chunk = [] {
+        const char *e = getenv("MSOSM_FOLD_CHUNK");
+        long long c = e ? atoll(e) : 16384;
+        return c >= 1024 ? c : 16384;
+    }();
     const long long step = (long long)kFoldThreads * kFoldUnroll;
-    long long chunk = 16384;
+    long long chunk = min_chunk;
     while ((long long)(n + chunk - 1) / chunk > 4096)
         chunk *= 2;
     chunk = (chunk + step - 1) / step * step;

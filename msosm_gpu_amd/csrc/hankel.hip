@@ -35,6 +35,7 @@
 
 #include <climits>
 #include <cstdint>
+#include <type_traits>
 
 #include "fft_device.h"
 #include "kernels.h"
@@ -155,22 +156,37 @@ __global__ __launch_bounds__(64 * kWaves) void k_hankel(const void *__restrict__
     // ring columns of this workgroup's bins: u -> absolute block
     // a = g0 + gb + kGTile - 1 - u; zero before the stream start and past the
     // oldest block any (g, k) of the tile uses
-    for (int idx = tid; idx < NBIN * U; idx += blockDim.x) {
-        const int nl = idx % NBIN, u = idx / NBIN;
-        const long long a = g0 + gb + kGTile - 1 - u;
-        float2 x = make_float2(0.f, 0.f);
-        if (a >= 0 && u < kGTile - 1 + kmax) {
-            const long long slot = a % rx;
-            x = Md::load(ring + slot * N + n0 + nl);
+    // (batches of kLoadBatch independent loads per thread, issued before any
+    // is converted, so they are in flight together)
+    constexpr int kLoadBatch = 8;
+    for (int base = 0; base < NBIN * U; base += kLoadBatch * 64 * kWaves) {
+        RingT raw[kLoadBatch];
+        bool use[kLoadBatch];
+#pragma unroll
+        for (int r = 0; r < kLoadBatch; r++) {
+            const int idx = base + r * 64 * kWaves + tid;
+            const int nl = idx % NBIN, u = idx / NBIN;
+            const long long a = g0 + gb + kGTile - 1 - u;
+            use[r] = idx < NBIN * U && a >= 0 && u < kGTile - 1 + kmax;
+            const long long slot = use[r] ? a % rx : 0;
+            raw[r] = ring[slot * N + n0 + nl];
         }
-        Op rh, rl, ih, il;
-        split<MODE>(x.x, rh, rl);
-        split<MODE>(x.y, ih, il);
-        z[(0 * NBIN + nl) * U + u] = rh;
-        z[(1 * NBIN + nl) * U + u] = ih;
-        if constexpr (P == 4) {
-            z[(2 * NBIN + nl) * U + u] = rl;
-            z[(3 * NBIN + nl) * U + u] = il;
+#pragma unroll
+        for (int r = 0; r < kLoadBatch; r++) {
+            const int idx = base + r * 64 * kWaves + tid;
+            if (idx >= NBIN * U)
+                break;
+            const int nl = idx % NBIN, u = idx / NBIN;
+            const float2 x = use[r] ? Md::load(&raw[r]) : make_float2(0.f, 0.f);
+            Op rh, rl, ih, il;
+            split<MODE>(x.x, rh, rl);
+            split<MODE>(x.y, ih, il);
+            z[(0 * NBIN + nl) * U + u] = rh;
+            z[(1 * NBIN + nl) * U + u] = ih;
+            if constexpr (P == 4) {
+                z[(2 * NBIN + nl) * U + u] = rl;
+                z[(3 * NBIN + nl) * U + u] = il;
+            }
         }
     }
 
@@ -181,17 +197,29 @@ __global__ __launch_bounds__(64 * kWaves) void k_hankel(const void *__restrict__
     const int ntiles = (ndm + kDmTile - 1) / kDmTile;
     for (int tile = 0; tile < ntiles; tile++) {
         __syncthreads(); // z ready (first tile); tables and staging free
-        for (int idx = tid; idx < kDmTile * NBIN; idx += blockDim.x) {
-            const int i = idx / NBIN, nl = idx % NBIN;
-            const int dm = tile * kDmTile + i;
-            int d = -1;
-            float2 h = make_float2(0.f, 0.f);
-            if (dm < ndm) {
-                d = delay[(long long)dm * N + n0 + nl];
-                h = Md::load(H + (long long)dm * N + n0 + nl);
+        {
+            // kDmTile * NBIN = 512 (FP16) or 256 entries: at most 2 per thread,
+            // both loads of both entries issued before use
+            constexpr int PER = (kDmTile * NBIN + 64 * kWaves - 1) / (64 * kWaves);
+            uint16_t dr[PER];
+            RingT hr[PER];
+#pragma unroll
+            for (int r = 0; r < PER; r++) {
+                const int idx = tid + r * 64 * kWaves;
+                const int i = idx / NBIN, nl = idx % NBIN;
+                const int dm = min(tile * kDmTile + i, ndm - 1);
+                dr[r] = delay[(long long)dm * N + n0 + nl];
+                hr[r] = H[(long long)dm * N + n0 + nl];
             }
-            dd[idx] = d;
-            hh[idx] = h;
+#pragma unroll
+            for (int r = 0; r < PER; r++) {
+                const int idx = tid + r * 64 * kWaves;
+                if (idx < kDmTile * NBIN) {
+                    const bool ok = tile * kDmTile + idx / NBIN < ndm;
+                    dd[idx] = ok ? (int)dr[r] : -1;
+                    hh[idx] = ok ? Md::load(&hr[r]) : make_float2(0.f, 0.f);
+                }
+            }
         }
         __syncthreads();
         for (int nl = wave; nl < NBIN; nl += kWaves) {
@@ -272,14 +300,27 @@ __global__ __launch_bounds__(fft_threads(LOG2N)) void k_inv_dense(const void *__
     const int tl = threadIdx.x / S::NT, t = threadIdx.x % S::NT;
     const long long u = (long long)blockIdx.x * TPB + tl;
     const bool valid = u < units;
+    // all E loads are issued before any is used (one guarded block, so the
+    // compiler keeps them in flight together instead of load-wait-convert)
+    using Raw = typename std::conditional<F16Y, __half2, float2>::type;
+    const Raw *src = reinterpret_cast<const Raw *>(Y_) + (valid ? u : 0) * N + t;
+    Raw raw[E];
+    if (valid) {
+#pragma unroll
+        for (int m = 0; m < E; m++)
+            raw[m] = src[m * S::NT];
+    } else {
+#pragma unroll
+        for (int m = 0; m < E; m++)
+            raw[m] = Raw{};
+    }
     float2 v[E];
 #pragma unroll
     for (int m = 0; m < E; m++) {
-        const long long idx = u * N + t + m * S::NT;
         if constexpr (F16Y)
-            v[m] = valid ? __half22float2(reinterpret_cast<const __half2 *>(Y_)[idx]) : make_float2(0.f, 0.f);
+            v[m] = __half22float2(raw[m]);
         else
-            v[m] = valid ? reinterpret_cast<const float2 *>(Y_)[idx] : make_float2(0.f, 0.f);
+            v[m] = raw[m];
     }
     fft_block<LOG2N, E, true>(v, smem + tl * S::NPAD, twlo, twhi, t);
     if (valid) {

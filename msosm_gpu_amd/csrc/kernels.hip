@@ -419,6 +419,21 @@ __global__ __launch_bounds__(fft_threads(LOG2N)) void k_fwd_u16(
     }
 }
 
+template <bool F16>
+struct RingElem;
+template <>
+struct RingElem<true> {
+    using T = __half2;
+    __device__ static float2 f2(T x) { return __half22float2(x); }
+    __device__ static T zero() { return __floats2half2_rn(0.f, 0.f); }
+};
+template <>
+struct RingElem<false> {
+    using T = float2;
+    __device__ static float2 f2(T x) { return x; }
+    __device__ static T zero() { return make_float2(0.f, 0.f); }
+};
+
 // Fused inverse: unit u = (dm, block b) with blocks of one DM consecutive.
 // Y[n] = X_{g-d(n)}[n] * H[n] (zero before the stream start), inverse FFT,
 // keep points [M, 2M), scale: FP32 ring/H -> 1/N^2 (the reference's two 1/N
@@ -445,28 +460,29 @@ __global__ __launch_bounds__(fft_threads(LOG2N)) void k_inv_fused(
     const long long g = g0 + b;
     const int gslot = (int)(g % rx);
     const uint16_t *drow = delay + (long long)dm * N;
-    float2 v[E];
+    // three batches of independent loads (delays, then ring and H values) so
+    // each batch is in flight at once; a ring index is clamped to slot 0 when
+    // its block precedes the stream start and the value is dropped after
+    using R = RingElem<F16>;
+    using T = typename R::T;
+    int d[E];
+#pragma unroll
+    for (int m = 0; m < E; m++)
+        d[m] = drow[t + m * S::NT];
+    T xr[E], hr[E];
+    const T *ringT = reinterpret_cast<const T *>(ring);
+    const T *hrow = reinterpret_cast<const T *>(H) + (long long)dm * N;
 #pragma unroll
     for (int m = 0; m < E; m++) {
-        const int n = t + m * S::NT;
-        float2 x = make_float2(0.f, 0.f);
-        const int d = drow[n];
-        if (valid && g >= d) {
-            int sl = gslot - d;
-            sl += sl < 0 ? rx : 0;
-            const long long off = (long long)sl * N + n;
-            if constexpr (F16) {
-                float2 xs = __half22float2(reinterpret_cast<const __half2 *>(ring)[off]);
-                float2 h = __half22float2(reinterpret_cast<const __half2 *>(H)[(long long)dm * N + n]);
-                x = cmul(xs, h);
-            } else {
-                float2 xs = reinterpret_cast<const float2 *>(ring)[off];
-                float2 h = reinterpret_cast<const float2 *>(H)[(long long)dm * N + n];
-                x = cmul(xs, h);
-            }
-        }
-        v[m] = x;
+        int sl = gslot - d[m];
+        sl += sl < 0 ? rx : 0;
+        xr[m] = ringT[(long long)sl * N + t + m * S::NT];
+        hr[m] = hrow[t + m * S::NT];
     }
+    float2 v[E];
+#pragma unroll
+    for (int m = 0; m < E; m++)
+        v[m] = (valid && g >= d[m]) ? cmul(R::f2(xr[m]), R::f2(hr[m])) : make_float2(0.f, 0.f);
     fft_block<LOG2N, E, true>(v, smem + tl * S::NPAD, twlo, twhi, t);
     if (valid) {
         const float scale = F16 ? 0.5f : 1.0f / ((float)N * (float)N);
@@ -476,21 +492,6 @@ __global__ __launch_bounds__(fft_threads(LOG2N)) void k_inv_fused(
             dst[t + m * S::NT - M] = cscale(v[m], scale);
     }
 }
-
-template <bool F16>
-struct RingElem;
-template <>
-struct RingElem<true> {
-    using T = __half2;
-    __device__ static float2 f2(T x) { return __half22float2(x); }
-    __device__ static T zero() { return __floats2half2_rn(0.f, 0.f); }
-};
-template <>
-struct RingElem<false> {
-    using T = float2;
-    __device__ static float2 f2(T x) { return x; }
-    __device__ static T zero() { return make_float2(0.f, 0.f); }
-};
 
 // Fused inverse for one transform per workgroup (2^12..2^14 points): the
 // workgroup runs NB consecutive blocks of one DM trial and issues the gather
