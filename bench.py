@@ -156,6 +156,8 @@ def main():
                     traffic=measured_traffic(args, len(dms)),
                     alg_bytes_per_launch=alg_bytes, ms_per_launch=st["ms_per_launch"])
     kernels = kernel_rooflines(results, plan_info, len(dms), fold, hankel_flops)
+    copy_gbs = copy_bandwidth(torch, dev)
+    log(f"device copy (1 GiB read + 1 GiB write): {copy_gbs:.0f} GB/s")
     kernels["k_inv_fused<14,fp16>"] = {k: roofline[k] for k in ("achieved", "unit", "peak", "frac", "ms_per_launch")}
     cpu = None
     if not args.no_cpu_baseline:
@@ -188,6 +190,7 @@ def main():
         "ratio_mixed_vs_plain_f32": (round(mixed["value"] / results["plain_f32"]["value"], 3)
                                      if "plain_f32" in results else None),
         "kernels": kernels,
+        "hbm_copy_gbs": round(copy_gbs, 1),
         "rel_l2_vs_fp64": {k: v for k, v in checks.items()},
     }
     print(json.dumps(line), flush=True)
@@ -232,6 +235,24 @@ def pmc_workload(ms, torch, dev, bw, f0, dms, args, calls=6):
     torch.cuda.synchronize()
     print(json.dumps({"pmc_workload": {"calls": calls, "units_per_call": len(dms) * plan.count,
                                        "copy_bytes": src.numel() * 4}}), flush=True)
+
+
+def copy_bandwidth(torch, dev, reps=5):
+    """Achievable HBM bandwidth on this device: a 1 GiB device-to-device copy,
+    (read + write bytes) / time, best of reps."""
+    src = torch.ones(1 << 28, dtype=torch.float32, device=dev)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    best = 0.0
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        dst.copy_(src)
+        e1.record()
+        torch.cuda.synchronize()
+        best = max(best, 2 * src.numel() * 4 / (e0.elapsed_time(e1) / 1e3) / 1e9)
+    del src, dst
+    return best
 
 
 def fold_bench(ms, torch, eng, plan, stream, period_samples, nbins, launches=20):

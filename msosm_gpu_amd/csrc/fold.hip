@@ -79,14 +79,15 @@ __global__ void k_fold_bins(float *__restrict__ folded, const float *__restrict_
 // global profile once.  value = sqrt(|a|^2+|b|^2) (the reference's
 // intensity, gpu_kernel.cu:352) or |a|^2+|b|^2 (power).
 //
-// Each workgroup folds one contiguous chunk of its trial's samples, so it
-// touches only the bins its chunk spans and adds only those (non-zero) bins
-// to the global profile.  The loop is wave-uniform: every wave takes 8
-// groups of 64 consecutive samples per step (8 loads in flight), idle lanes
-// contribute nothing.  When a period spans many samples per bin, all lanes
-// of a group usually share one bin: the wave then sums its values with lane
-// shuffles and one lane issues a single LDS atomic instead of 64
-// conflicting ones.
+// Each workgroup folds one contiguous chunk of its trial's samples, split
+// into one contiguous piece per wave, so it touches only the bins its chunk
+// spans and adds only those (non-zero) bins to the global profile.  A wave
+// reads 8 groups of 64 consecutive samples per step (8 loads in flight).
+// While every sample of a group falls into the bin the wave is accumulating
+// (the usual case when a bin spans many samples), each lane just adds its
+// value to a register; when the bin changes the wave sums its registers with
+// lane shuffles and one lane adds the total to the LDS histogram.  Groups that
+// straddle bins fall back to one LDS atomic per lane.
 __device__ __forceinline__ float wave_sum(float x)
 {
 #pragma unroll
@@ -114,47 +115,70 @@ __global__ __launch_bounds__(kFoldThreads) void k_fold_phase(const float2 *__res
     const float2 *ad = a + (long long)dm * stride;
     const float2 *bd = b ? b + (long long)dm * stride : nullptr;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const long long c0 = (long long)blockIdx.x * chunk;
-    const long long c1 = c0 + chunk < n ? c0 + chunk : n;
     constexpr int waves = kFoldThreads / 64;
-    for (long long base = c0 + (long long)wave * 64 * kFoldUnroll; base < c1; base += waves * 64 * kFoldUnroll) {
+    constexpr int group_step = 64 * kFoldUnroll;
+    // chunk is a multiple of waves * group_step: equal contiguous pieces
+    const long long piece = chunk / waves;
+    const long long w0 = (long long)blockIdx.x * chunk + wave * piece;
+    const long long w1 = min(w0 + piece, n);
+    float acc = 0.f, cnt = 0.f;
+    int cur = -1; // wave-uniform bin being accumulated in registers
+    auto flush = [&]() {
+        if (cur >= 0) {
+            const float sum = wave_sum(acc);
+            const float c = count_hits ? wave_sum(cnt) : 0.f;
+            if (lane == 0) {
+                atomicAdd(&hist[cur], sum);
+                if (count_hits)
+                    atomicAdd(&hist[nbins + cur], c);
+            }
+        }
+        acc = 0.f;
+        cnt = 0.f;
+        cur = -1;
+    };
+    // Index arithmetic per step is uniform (scalar); per lane only the offset
+    // u*64 + lane varies.  The phase is frac((n0 + i) / P) in FP64.
+    for (long long base = w0; base < w1; base += group_step) { // wave-uniform bound
+        const long long rem = w1 - base; // samples left for this wave, uniform
+        const float2 *ga = ad + base;
+        const float2 *gb = bd ? bd + base : nullptr;
         float p[kFoldUnroll];
 #pragma unroll
         for (int u = 0; u < kFoldUnroll; u++) {
-            const long long i = base + u * 64 + lane;
+            const int off = u * 64 + lane;
             p[u] = 0.f;
-            if (i < c1) {
-                const float2 v = ad[i];
+            if (off < rem) {
+                const float2 v = ga[off];
                 p[u] = v.x * v.x + v.y * v.y;
-                if (bd) {
-                    const float2 w = bd[i];
+                if (gb) {
+                    const float2 w = gb[off];
                     p[u] += w.x * w.x + w.y * w.y;
                 }
             }
         }
 #pragma unroll
         for (int u = 0; u < kFoldUnroll; u++) {
-            const long long i = base + u * 64 + lane;
-            const bool valid = i < c1;
+            if (u * 64 >= rem) // uniform: nothing left in this group or after
+                break;
+            const bool valid = u * 64 + lane < rem;
             float v = p[u];
             if (!power)
                 v = sqrtf(v);
-            int bin = 0;
-            if (valid) {
-                double ph = (double)(n0 + i) * inv_period;
-                ph -= floor(ph);
-                bin = (int)(ph * (double)nbins);
-                bin = bin < nbins ? bin : nbins - 1;
+            double ph = (double)(n0 + base + u * 64 + lane) * inv_period;
+            ph -= floor(ph);
+            int bin = (int)(ph * (double)nbins);
+            bin = bin < nbins ? bin : nbins - 1;
+            bin = valid ? bin : -1;
+            const int b0 = __builtin_amdgcn_readfirstlane(bin); // lane 0 is valid here
+            const bool uni = __all(!valid || bin == b0);       // uniform
+            if (!uni || b0 != cur) {                            // uniform
+                flush();
+                cur = uni ? b0 : -1;
             }
-            const int b0 = __shfl(bin, 0);
-            if (__all(!valid || bin == b0)) {
-                const float sum = wave_sum(v);
-                const float cnt = count_hits ? wave_sum(valid ? 1.0f : 0.0f) : 0.0f;
-                if (lane == 0 && base + u * 64 < c1) {
-                    atomicAdd(&hist[b0], sum);
-                    if (count_hits)
-                        atomicAdd(&hist[nbins + b0], cnt);
-                }
+            if (uni) {
+                acc += v;
+                cnt += valid ? 1.0f : 0.0f;
             } else if (valid) {
                 atomicAdd(&hist[bin], v);
                 if (count_hits)
@@ -162,6 +186,7 @@ __global__ __launch_bounds__(kFoldThreads) void k_fold_phase(const float2 *__res
             }
         }
     }
+    flush();
     __syncthreads();
     for (int i = threadIdx.x; i < nbins; i += blockDim.x) {
         const float h = hist[i];
@@ -537,7 +562,7 @@ int msosm_fold_phase(const void *a, const void *b, size_t n, int ndm, size_t str
         long long c = e ? atoll(e) : 16384;
         return c >= 1024 ? c : 16384;
     }();
-    const long long step = (long long)kFoldThreads * kFoldUnroll;
+    const long long step = (long long)(kFoldThreads / 64) * 64 * kFoldUnroll; // one group step of every wave
     long long chunk = min_chunk;
     while ((long long)(n + chunk - 1) / chunk > 4096)
         chunk *= 2;

@@ -158,7 +158,14 @@ __global__ __launch_bounds__(64 * kWaves) void k_hankel(const void *__restrict__
     // oldest block any (g, k) of the tile uses
     // (batches of kLoadBatch independent loads per thread, issued before any
     // is converted, so they are in flight together)
+    // 32-bit slot arithmetic: slot(u) = (s0 - u) mod rx with s0 the slot of
+    // block g0 + gb + kGTile - 1; u < kGTile - 1 + kmax <= rx + kGTile - 1, so
+    // at most two wraps.  Blocks before the stream start (u > g0 + gb +
+    // kGTile - 1) read as zero.
     constexpr int kLoadBatch = 8;
+    const long long top = g0 + gb + kGTile - 1;
+    const int s0 = (int)(top % rx);
+    const int ulim = (int)min(top + 1, (long long)(kGTile - 1 + kmax)); // loaded u < ulim
     for (int base = 0; base < NBIN * U; base += kLoadBatch * 64 * kWaves) {
         RingT raw[kLoadBatch];
         bool use[kLoadBatch];
@@ -166,10 +173,11 @@ __global__ __launch_bounds__(64 * kWaves) void k_hankel(const void *__restrict__
         for (int r = 0; r < kLoadBatch; r++) {
             const int idx = base + r * 64 * kWaves + tid;
             const int nl = idx % NBIN, u = idx / NBIN;
-            const long long a = g0 + gb + kGTile - 1 - u;
-            use[r] = idx < NBIN * U && a >= 0 && u < kGTile - 1 + kmax;
-            const long long slot = use[r] ? a % rx : 0;
-            raw[r] = ring[slot * N + n0 + nl];
+            use[r] = idx < NBIN * U && u < ulim;
+            int sl = s0 - (use[r] ? u : 0);
+            sl += sl < 0 ? rx : 0;
+            sl += sl < 0 ? rx : 0;
+            raw[r] = ring[(size_t)sl * N + n0 + nl];
         }
 #pragma unroll
         for (int r = 0; r < kLoadBatch; r++) {
@@ -195,33 +203,37 @@ __global__ __launch_bounds__(64 * kWaves) void k_hankel(const void *__restrict__
     const int kq = 8 * (q & 1);
     const bool imag_half = q >= 2;
     const int ntiles = (ndm + kDmTile - 1) / kDmTile;
+    // one DM tile's delays and filter values, kDmTile * NBIN entries: at most
+    // PER per thread, loaded one tile ahead so the loads overlap the MFMA
+    // work of the current tile
+    constexpr int PER = (kDmTile * NBIN + 64 * kWaves - 1) / (64 * kWaves);
+    uint16_t dr[PER];
+    RingT hr[PER];
+    auto load_tables = [&](int tl) {
+#pragma unroll
+        for (int r = 0; r < PER; r++) {
+            const int idx = tid + r * 64 * kWaves;
+            const int i = idx / NBIN, nl = idx % NBIN;
+            const int dm = min(tl * kDmTile + i, ndm - 1);
+            dr[r] = delay[(long long)dm * N + n0 + nl];
+            hr[r] = H[(long long)dm * N + n0 + nl];
+        }
+    };
+    load_tables(0);
     for (int tile = 0; tile < ntiles; tile++) {
         __syncthreads(); // z ready (first tile); tables and staging free
-        {
-            // kDmTile * NBIN = 512 (FP16) or 256 entries: at most 2 per thread,
-            // both loads of both entries issued before use
-            constexpr int PER = (kDmTile * NBIN + 64 * kWaves - 1) / (64 * kWaves);
-            uint16_t dr[PER];
-            RingT hr[PER];
 #pragma unroll
-            for (int r = 0; r < PER; r++) {
-                const int idx = tid + r * 64 * kWaves;
-                const int i = idx / NBIN, nl = idx % NBIN;
-                const int dm = min(tile * kDmTile + i, ndm - 1);
-                dr[r] = delay[(long long)dm * N + n0 + nl];
-                hr[r] = H[(long long)dm * N + n0 + nl];
-            }
-#pragma unroll
-            for (int r = 0; r < PER; r++) {
-                const int idx = tid + r * 64 * kWaves;
-                if (idx < kDmTile * NBIN) {
-                    const bool ok = tile * kDmTile + idx / NBIN < ndm;
-                    dd[idx] = ok ? (int)dr[r] : -1;
-                    hh[idx] = ok ? Md::load(&hr[r]) : make_float2(0.f, 0.f);
-                }
+        for (int r = 0; r < PER; r++) {
+            const int idx = tid + r * 64 * kWaves;
+            if (idx < kDmTile * NBIN) {
+                const bool ok = tile * kDmTile + idx / NBIN < ndm;
+                dd[idx] = ok ? (int)dr[r] : -1;
+                hh[idx] = ok ? Md::load(&hr[r]) : make_float2(0.f, 0.f);
             }
         }
         __syncthreads();
+        if (tile + 1 < ntiles)
+            load_tables(tile + 1);
         for (int nl = wave; nl < NBIN; nl += kWaves) {
             const int d = dd[row * NBIN + nl];
             const float2 h = hh[row * NBIN + nl];
