@@ -241,10 +241,12 @@ int msosm_batch_destroy(msosm_batch *b);
  * [min_fftpoint, max_fftpoint] that are not below the plan's minimum, time
  * `calls` filter calls of about target_samples samples on synthetic input and
  * return the FFT size with the highest DM-trial*sample throughput (results are
- * cached per descriptor for the life of the process). */
+ * cached per descriptor for the life of the process).  Sizes above 2^14 are
+ * timed with MSOSM_FORM_PLAIN_F32 when desc->form is a single-workgroup form;
+ * *best_form (may be NULL) receives the form the chosen size runs with. */
 int msosm_autotune_fftpoint(const msosm_plan_desc *desc, unsigned long min_fftpoint, unsigned long max_fftpoint,
                             unsigned long target_samples, int calls, void *stream, unsigned long *best_fftpoint,
-                            double *best_ns_per_dm_sample);
+                            double *best_ns_per_dm_sample, int *best_form);
 
 /* ---- learned DM proposal (north-star component; no reference counterpart:
  * the reference dedisperses every trial of a grid,

@@ -112,7 +112,7 @@ SIGNATURES = {
     "msosm_batch_profiles": (_i, [_vp, _i, _vp, _vp]),
     "msosm_batch_destroy": (_i, [_vp]),
     "msosm_autotune_fftpoint": (_i, [ctypes.POINTER(PlanDesc), _ul, _ul, _ul, _i, _vp, ctypes.POINTER(_ul),
-                                     ctypes.POINTER(ctypes.c_double)]),
+                                     ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_i)]),
     "msosm_sim_pulsar": (_i, [_vp, _sz, ctypes.c_double, ctypes.c_double, _f, ctypes.c_uint, _vp]),
     "msosm_sim_disperse": (_i, [_vp, _sz, ctypes.c_double, ctypes.c_double, ctypes.c_double, _i, _vp]),
     "msosm_sim_quantize": (_i, [_vp, _sz, _f, ctypes.c_uint, _f, _vp, _vp]),
@@ -456,16 +456,18 @@ class Batch:
 
 def autotune_fftpoint(bw, dm, f0, min_fftpoint=1024, max_fftpoint=65536, target_samples=1 << 20, calls=3,
                       method=METHOD_MSOSM, chirp=CHIRP_REF_F32W, form=FORM_FUSED_F16, stream=None):
-    """Measured best FFT size for these DM trials; returns (fftpoint, ns per DM-trial sample)."""
+    """Measured best FFT size for these DM trials; returns (fftpoint, ns per
+    DM-trial sample, form the size runs with)."""
     n = max(np.size(bw), np.size(dm), np.size(f0))
     bwa, dma, f0a = _farr(bw, n), _farr(dm, n), _farr(f0, n)
     fp = ctypes.POINTER(ctypes.c_float)
     d = PlanDesc(n, bwa.ctypes.data_as(fp), dma.ctypes.data_as(fp), f0a.ctypes.data_as(fp), 0, 1, 0, method,
                  chirp, form)
-    best, ns = ctypes.c_ulong(), ctypes.c_double()
+    best, ns, bf = ctypes.c_ulong(), ctypes.c_double(), ctypes.c_int()
     check(lib().msosm_autotune_fftpoint(ctypes.byref(d), min_fftpoint, max_fftpoint, target_samples, calls,
-                                        _stream_handle(stream), ctypes.byref(best), ctypes.byref(ns)))
-    return int(best.value), float(ns.value)
+                                        _stream_handle(stream), ctypes.byref(best), ctypes.byref(ns),
+                                        ctypes.byref(bf)))
+    return int(best.value), float(ns.value), int(bf.value)
 
 
 def fold_phase(a, n, period_samples, nbins, prof, hits=None, b=None, ndm=1, stride=None, n0=0, power=True,

@@ -255,11 +255,24 @@ int msosm_batch_destroy(msosm_batch *b)
 // ---------------------------------------------------------------------------
 // Autotuner: measured choice of the FFT size.
 
+static int ilog2_fp(unsigned long v)
+{
+    int l = 0;
+    while ((1ul << (l + 1)) <= v)
+        l++;
+    return l;
+}
+
 int msosm_autotune_fftpoint(const msosm_plan_desc *desc, unsigned long min_fftpoint, unsigned long max_fftpoint,
                             unsigned long target_samples, int calls, void *stream, unsigned long *best_fftpoint,
-                            double *best_ns_per_dm_sample)
+                            double *best_ns_per_dm_sample, int *best_form)
 {
-    static std::map<std::string, std::pair<unsigned long, double>> cache;
+    struct Choice {
+        unsigned long fftpoint;
+        double ns;
+        int form;
+    };
+    static std::map<std::string, Choice> cache;
     if (!desc || !best_fftpoint || calls < 1 || target_samples < 1)
         return msosm::set_error(MSOSM_ERR_ARG, "autotune: bad argument");
     std::string key;
@@ -275,25 +288,48 @@ int msosm_autotune_fftpoint(const msosm_plan_desc *desc, unsigned long min_fftpo
     }
     auto it = cache.find(key);
     if (it != cache.end()) {
-        *best_fftpoint = it->second.first;
+        *best_fftpoint = it->second.fftpoint;
         if (best_ns_per_dm_sample)
-            *best_ns_per_dm_sample = it->second.second;
+            *best_ns_per_dm_sample = it->second.ns;
+        if (best_form)
+            *best_form = it->second.form;
         return MSOSM_OK;
     }
     hipStream_t s = (hipStream_t)stream;
     unsigned long best = 0;
     double best_ns = 1e300;
+    int best_f = desc->form;
     uint32_t *input = nullptr;
     size_t input_words = 0;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     BHIP(hipEventCreate(&e0));
     BHIP(hipEventCreate(&e1));
+    // smallest size the trials allow (a plan with fftpoint 0 takes it without
+    // the below-minimum warning)
+    unsigned long fp_min = 4;
+    {
+        msosm_plan_desc d = *desc;
+        d.fftpoint = 0;
+        d.input_size = 0;
+        d.count = 1;
+        msosm_plan *plan = nullptr;
+        if (msosm_plan_create(&d, &plan) == MSOSM_OK) {
+            msosm_plan_info info{};
+            msosm_plan_get_info(plan, &info);
+            fp_min = info.fftpoint;
+            msosm_plan_destroy(plan);
+        }
+    }
     for (unsigned long fp = 4; fp <= max_fftpoint && fp <= (1ul << 28); fp <<= 1) {
-        if (fp < min_fftpoint)
+        if (fp < min_fftpoint || fp < fp_min)
             continue;
         msosm_plan_desc d = *desc;
         d.fftpoint = fp;
         d.input_size = 0;
+        // the single-workgroup forms stop at 2^14 points; above that the size
+        // is measured with the reference-structured form
+        if (d.form != MSOSM_FORM_PLAIN_F32 && !msosm::fused_supported(ilog2_fp(fp)))
+            d.form = MSOSM_FORM_PLAIN_F32;
         d.count = (int)std::max(1ul, target_samples / (fp / 2));
         msosm_plan *plan = nullptr;
         if (msosm_plan_create(&d, &plan) != MSOSM_OK)
@@ -334,6 +370,7 @@ int msosm_autotune_fftpoint(const msosm_plan_desc *desc, unsigned long min_fftpo
             if (ns < best_ns) {
                 best_ns = ns;
                 best = fp;
+                best_f = d.form;
             }
         }
         msosm_engine_destroy(eng);
@@ -347,10 +384,12 @@ int msosm_autotune_fftpoint(const msosm_plan_desc *desc, unsigned long min_fftpo
     if (!best)
         return msosm::set_error(MSOSM_ERR_UNSUPPORTED, "autotune: no FFT size in [%lu, %lu] fits these DM trials",
                                 min_fftpoint, max_fftpoint);
-    cache[key] = {best, best_ns};
+    cache[key] = {best, best_ns, best_f};
     *best_fftpoint = best;
     if (best_ns_per_dm_sample)
         *best_ns_per_dm_sample = best_ns;
+    if (best_form)
+        *best_form = best_f;
     return MSOSM_OK;
 }
 

@@ -78,11 +78,19 @@ def test_batch_graph_matches_eager(device):
 
 def test_autotune_picks_a_valid_size(device):
     bw, f0, dms = 64e6, 1368e6, [20.0, 80.0, 300.0]
-    best, ns = ms.autotune_fftpoint(bw, dms, f0, min_fftpoint=1024, max_fftpoint=16384, target_samples=1 << 19,
-                                    calls=2)
+    best, ns, form = ms.autotune_fftpoint(bw, dms, f0, min_fftpoint=1024, max_fftpoint=16384,
+                                          target_samples=1 << 19, calls=2)
     mmin = max(ms.dm_model(bw, d, f0)[1] for d in dms)
     assert best & (best - 1) == 0 and 2 * mmin <= best <= 16384
-    assert ns > 0
+    assert ns > 0 and form == ms.FORM_FUSED_F16
     again = ms.autotune_fftpoint(bw, dms, f0, min_fftpoint=1024, max_fftpoint=16384, target_samples=1 << 19,
                                  calls=2)
-    assert again == (best, ns)  # cached
+    assert again == (best, ns, form)  # cached
+
+
+def test_autotune_above_single_workgroup_sizes(device):
+    # sizes above 2^14 are timed with the reference-structured form
+    bw, f0 = 128e6, 1336e6
+    best, ns, form = ms.autotune_fftpoint(bw, [100.0], f0, min_fftpoint=32768, max_fftpoint=65536,
+                                          target_samples=1 << 20, calls=1)
+    assert best in (32768, 65536) and form == ms.FORM_PLAIN_F32 and ns > 0
