@@ -290,6 +290,13 @@ int msosm_adam(float *p, void *p16, const float *g, float *m, float *v, size_t n
                float eps, int step, void *stream);
 int msosm_f32_to_f16(const float *a, void *b, size_t n, void *stream);
 
+/* Page-locked host memory for input and output buffers (replaces the
+ * reference drivers' cudaHostRegister of their buffers, e.g.
+ * test_processing_speed_msosm.cpp:182,196, which has no counterpart on this
+ * platform for existing allocations).  msosm_host_free(NULL) is a no-op. */
+int msosm_host_alloc(size_t bytes, void **out);
+int msosm_host_free(void *p);
+
 const char *msosm_last_error(void);
 const char *msosm_version(void);
 

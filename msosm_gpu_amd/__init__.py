@@ -91,6 +91,8 @@ SIGNATURES = {
     "msosm_k_multiply_delay_batch": (_i, [_vp, _vp, _i, _vp, _vp, _i, _i, _i, _i, _vp]),
     "msosm_k_discard_batch": (_i, [_vp, _vp, _i, _i, _i, _vp]),
     "msosm_k_fft_c2c": (_i, [_vp, _vp, _sz, _sz, _sz, _sz, _i, _vp]),
+    "msosm_host_alloc": (_i, [_sz, ctypes.POINTER(_vp)]),
+    "msosm_host_free": (_i, [_vp]),
     "msosm_k_hankel_delay_gemm": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _i, _ll, _i, _i, _vp]),
     "msosm_k_inverse_discard": (_i, [_vp, _vp, _i, _i, _sz, _f, _vp]),
     "msosm_fold_create": (_i, [_f, _f, _ul, _ul, _vp, ctypes.POINTER(_vp)]),

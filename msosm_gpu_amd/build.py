@@ -98,6 +98,20 @@ def build_facade_check():
     return out
 
 
+def build_apps():
+    """Native programs of the package (msosm_gpu_amd/apps -> msosm_gpu_amd/bin)."""
+    outs = []
+    bindir = os.path.join(ROOT, "msosm_gpu_amd", "bin")
+    os.makedirs(bindir, exist_ok=True)
+    for name in ("msosm_psrdada",):
+        src = os.path.join(ROOT, "msosm_gpu_amd", "apps", name + ".cpp")
+        out = os.path.join(bindir, name)
+        _run(["g++", "-std=c++17", "-O2", "-Wall", "-I", INCLUDE, src, "-o", out, "-L", LIBDIR, "-lmsosm",
+              "-Wl,-rpath,$ORIGIN/../lib"])
+        outs.append(out)
+    return outs
+
+
 def clean():
     shutil.rmtree(OBJDIR, ignore_errors=True)
 
@@ -105,3 +119,4 @@ def clean():
 if __name__ == "__main__":
     print(build_library())
     print(build_facade_check())
+    print(build_apps())

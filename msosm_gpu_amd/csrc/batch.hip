@@ -356,7 +356,7 @@ int msosm_autotune_fftpoint(const msosm_plan_desc *desc, unsigned long min_fftpo
                 return msosm::set_error(MSOSM_ERR_ALLOC, "autotune: input allocation");
             }
             input_words = words;
-            msosm::launch_fill_random_u16(input, words, 0x5EED, msosm::engine_stream(eng));
+            (void)msosm::launch_fill_random_u16(input, words, 0x5EED, msosm::engine_stream(eng));
         }
         hipStream_t es = msosm::engine_stream(eng);
         int rc = msosm_engine_filter_u16(eng, input, 1); // warm-up

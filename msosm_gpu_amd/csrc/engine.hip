@@ -535,6 +535,22 @@ const msosm::Plan &msosm::engine_plan(msosm_engine *e) { return e->plan; }
 
 extern "C" {
 
+int msosm_host_alloc(size_t bytes, void **out)
+{
+    if (!out || bytes == 0)
+        return set_error(MSOSM_ERR_ARG, "host_alloc: bad argument");
+    *out = nullptr;
+    HIP_TRY(hipHostMalloc(out, bytes, hipHostMallocDefault));
+    return MSOSM_OK;
+}
+
+int msosm_host_free(void *p)
+{
+    if (p)
+        HIP_TRY(hipHostFree(p));
+    return MSOSM_OK;
+}
+
 const char *msosm_last_error(void) { return g_last_error.c_str(); }
 const char *msosm_version(void) { return "msosm-mi355x 0.1 (gfx950)"; }
 
