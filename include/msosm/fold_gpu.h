@@ -38,10 +38,14 @@ public:
     // fold_gpu.cpp:29-33
     void discard_samples(unsigned long Nd) { msosm_facade::check(msosm_fold_discard_samples(f_, Nd), "discard"); }
     // fold_gpu.cpp:35-43: intensity of two polarisations' device outputs,
-    // copied to total_intensity as the reference does
+    // copied to total_intensity as the reference does.  The engines run on
+    // their own streams, so their last calls are waited for first (the
+    // reference ran everything on one stream).
     template <typename T>
     void calculate_intensity(T *pol1, T *pol2)
     {
+        pol1->synchronize();
+        pol2->synchronize();
         msosm_facade::check(msosm_fold_intensity(f_, pol1->output_buffer_d, pol2->output_buffer_d), "intensity");
         msosm_facade::check(msosm_fold_get(f_, nullptr, nullptr, total_intensity), "intensity");
     }

@@ -112,7 +112,11 @@ int msosm_engine_create(const msosm_plan *plan, void *hip_stream, msosm_engine *
 int msosm_engine_destroy(msosm_engine *eng);
 /* One call: count*M uint16 pairs; where = 0 host pointer, 1 device pointer.
  * Replaces filter_block_uint16 (msosm_gpu_dm_concurrent.cpp:119-151,
- * msosm_gpu_batch.cpp:75-94, osm_gpu_batch.cpp:94-123). */
+ * msosm_gpu_batch.cpp:75-94, osm_gpu_batch.cpp:94-123).  A host buffer may be
+ * reused once the call returns; a device buffer must stay unchanged until the
+ * engine's stream has run the call.  The kernels run asynchronously on the
+ * engine's stream: work on other streams that reads the output must wait for
+ * it (msosm_engine_synchronize, or an event on msosm_engine_stream). */
 int msosm_engine_filter_u16(msosm_engine *eng, const void *input, int where);
 /* Output of the last call, layout [dm][count][M] complex64.
  * Replaces get_output(Complex*) (msosm_gpu_dm_concurrent.cpp:153-165). */
@@ -186,7 +190,9 @@ int msosm_fold_create(float period, float fs, unsigned long size, unsigned long 
 int msosm_fold_destroy(msosm_fold *f);
 int msosm_fold_discard_samples(msosm_fold *f, unsigned long nd);             /* :29-33 */
 /* Intensity of two polarisations' device outputs (size complex samples each);
- * pol2 may be NULL (single polarisation).  calculate_intensity, :35-43. */
+ * pol2 may be NULL (single polarisation).  calculate_intensity, :35-43.  Runs
+ * on the fold's stream: the engines that produced pol1/pol2 must have
+ * finished (or share that stream). */
 int msosm_fold_intensity(msosm_fold *f, const void *pol1_dev, const void *pol2_dev);
 /* Fold the current intensity buffer (fold_data, :48-173). */
 int msosm_fold_fold_data(msosm_fold *f);

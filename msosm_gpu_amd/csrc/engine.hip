@@ -113,6 +113,7 @@ struct msosm_engine {
     int kmax = 0;                   // largest delay + 1 over all trials
     long long g0 = 0;               // global index of the next block
     bool have_output = false;
+    hipEvent_t copy_done = nullptr; // host-input copy of the last call
     // timing
     bool timing = false;
     std::vector<StageTimer> pending;
@@ -187,6 +188,8 @@ void free_engine(msosm_engine *e)
     harvest(e);
     for (auto ev : e->pool)
         hipEventDestroy(ev);
+    if (e->copy_done)
+        hipEventDestroy(e->copy_done);
     void *bufs[] = {e->in_stage, e->H32, e->H16, e->delay32, e->delay16, e->delay_pk, e->out, e->out_u16, e->inbuf,
                     e->fftblk, e->ring, e->ifftblk, e->fft_scratch, e->tail, e->xring, e->ybuf};
     for (void *b : bufs)
@@ -647,9 +650,18 @@ int msosm_engine_filter_u16(msosm_engine *e, const void *input, int where)
     const msosm::Plan &p = e->plan;
     const uint32_t *cur = (const uint32_t *)input;
     if (where == 0) {
-        StageScope st(e, MSOSM_STAGE_COPY);
-        HIP_TRY(hipMemcpyAsync(e->in_stage, input, sizeof(uint32_t) * (size_t)p.count * p.M, hipMemcpyHostToDevice,
-                               e->stream));
+        // the caller may refill its host buffer as soon as this returns (the
+        // reference's blocking cudaMemcpy): wait for the copy, which in
+        // stream order also follows the previous call's kernels
+        {
+            StageScope st(e, MSOSM_STAGE_COPY);
+            HIP_TRY(hipMemcpyAsync(e->in_stage, input, sizeof(uint32_t) * (size_t)p.count * p.M,
+                                   hipMemcpyHostToDevice, e->stream));
+        }
+        if (!e->copy_done)
+            HIP_TRY(hipEventCreateWithFlags(&e->copy_done, hipEventDisableTiming));
+        HIP_TRY(hipEventRecord(e->copy_done, e->stream));
+        HIP_TRY(hipEventSynchronize(e->copy_done));
         cur = e->in_stage;
     } else if (where != 1) {
         return set_error(MSOSM_ERR_ARG, "filter_u16: where must be 0 (host) or 1 (device)");
