@@ -296,6 +296,26 @@ int msosm_adam(float *p, void *p16, const float *g, float *m, float *v, size_t n
                float eps, int step, void *stream);
 int msosm_f32_to_f16(const float *a, void *b, size_t n, void *stream);
 
+/* Output streaming to a host consumer thread (replaces the single host
+ * buffer + sentinel element + spin flags of get_output /
+ * get_output_thread, msosm_gpu_dm_concurrent.cpp:153-195, and the writer
+ * threads of multiple_dm_concurrent_msosm.cpp:84-146).  A ring of `slots`
+ * slots of slot_bytes, each a device staging buffer and a page-locked host
+ * buffer.  push (producer): the engine's last output (format 0 complex64,
+ * 1 uint16 as complexToUint16) is copied to the next free slot -- on the
+ * engine's stream into the staging buffer, so the engine's next call may
+ * start at once, then to the host on the ring's own stream; blocks while no
+ * slot is free (timeout_ms < 0: no limit), *seq = 0, 1, 2, ...  pop
+ * (consumer): waits for the oldest pushed slot's host copy, returns its data,
+ * byte count and sequence number; release frees it.  One producer and one
+ * consumer thread. */
+typedef struct msosm_outring msosm_outring;
+int msosm_outring_create(size_t slot_bytes, int slots, msosm_outring **out);
+int msosm_outring_destroy(msosm_outring *r);
+int msosm_outring_push(msosm_outring *r, msosm_engine *eng, int format, int timeout_ms, long long *seq);
+int msosm_outring_pop(msosm_outring *r, int timeout_ms, const void **data, size_t *bytes, long long *seq);
+int msosm_outring_release(msosm_outring *r);
+
 /* Page-locked host memory for input and output buffers (replaces the
  * reference drivers' cudaHostRegister of their buffers, e.g.
  * test_processing_speed_msosm.cpp:182,196, which has no counterpart on this

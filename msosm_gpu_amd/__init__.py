@@ -92,6 +92,11 @@ SIGNATURES = {
     "msosm_k_discard_batch": (_i, [_vp, _vp, _i, _i, _i, _vp]),
     "msosm_k_fft_c2c": (_i, [_vp, _vp, _sz, _sz, _sz, _sz, _i, _vp]),
     "msosm_host_alloc": (_i, [_sz, ctypes.POINTER(_vp)]),
+    "msosm_outring_create": (_i, [_sz, _i, ctypes.POINTER(_vp)]),
+    "msosm_outring_destroy": (_i, [_vp]),
+    "msosm_outring_push": (_i, [_vp, _vp, _i, _i, ctypes.POINTER(_ll)]),
+    "msosm_outring_pop": (_i, [_vp, _i, ctypes.POINTER(_vp), ctypes.POINTER(_sz), ctypes.POINTER(_ll)]),
+    "msosm_outring_release": (_i, [_vp]),
     "msosm_host_free": (_i, [_vp]),
     "msosm_k_hankel_delay_gemm": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _i, _ll, _i, _i, _vp]),
     "msosm_k_inverse_discard": (_i, [_vp, _vp, _i, _i, _sz, _f, _vp]),
@@ -412,6 +417,39 @@ def fft_c2c(inp, out, n, batch, idist=None, odist=None, direction=-1, stream=Non
     idist = n if idist is None else idist
     odist = n if odist is None else odist
     check(lib().msosm_k_fft_c2c(_ptr(inp), _ptr(out), n, batch, idist, odist, direction, _stream_handle(stream)))
+
+
+class OutRing:
+    """Ring of page-locked output slots between an engine (producer) and a
+    consumer thread; see msosm_outring_* in msosm_c.h."""
+
+    COMPLEX64, UINT16 = 0, 1
+
+    def __init__(self, slot_bytes, slots=3):
+        h = ctypes.c_void_p()
+        check(lib().msosm_outring_create(slot_bytes, slots, ctypes.byref(h)))
+        self.h = h
+
+    def push(self, engine, fmt=UINT16, timeout_ms=-1):
+        seq = ctypes.c_longlong()
+        check(lib().msosm_outring_push(self.h, engine.h, fmt, timeout_ms, ctypes.byref(seq)))
+        return seq.value
+
+    def pop(self, timeout_ms=-1):
+        """(sequence number, bytes object copy of the slot)."""
+        p, n, seq = ctypes.c_void_p(), ctypes.c_size_t(), ctypes.c_longlong()
+        check(lib().msosm_outring_pop(self.h, timeout_ms, ctypes.byref(p), ctypes.byref(n), ctypes.byref(seq)))
+        data = ctypes.string_at(p.value, n.value)
+        check(lib().msosm_outring_release(self.h))
+        return seq.value, data
+
+    def close(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.msosm_outring_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
 
 
 class Batch:
