@@ -38,7 +38,8 @@ int main(int argc, char **argv)
             return 2;
         }
     }
-    for (const char *s : {".batch.c64", ".conc.c64", ".conc.u16", ".osm.c64", ".loop.c64", ".fold.f32", ".inten.f32"})
+    for (const char *s : {".batch.c64", ".conc.c64", ".conc.u16", ".osm.c64", ".loop.c64", ".fold.f32", ".inten.f32",
+                          ".dmstream.c64", ".dmstream.u16", ".bstream0.c64", ".bstream1.c64", ".nosync.f32"})
         std::remove((pre + s).c_str());
     std::ofstream meta(pre + ".meta");
 
@@ -89,6 +90,50 @@ int main(int argc, char **argv)
             loop.filter_block_uint16(lp, i);
         for (int i = 0; i < 3; i++)
             dump(pre + ".loop.c64", lp[i], per);
+    }
+
+    // DM stream engine (msosm_gpu_batch_stream.h:52-93): trial i handed out
+    // with its own flag and a sentinel after its count*M samples
+    MSOSM_GPU_DM_BATCH_stream dms_eng(bws, dms, f0s, 3);
+    dms_eng.initialize_uint16(1024, 8);
+    std::vector<std::vector<Complex>> ds(3, std::vector<Complex>(per + 1));
+    std::vector<std::vector<uint16_pair>> du(3, std::vector<uint16_pair>(per + 1));
+    Complex *dsp[3] = {ds[0].data(), ds[1].data(), ds[2].data()};
+    uint16_pair *dup[3] = {du[0].data(), du[1].data(), du[2].data()};
+    for (unsigned long c = 0; c + per <= L; c += per) {
+        dms_eng.filter_block_uint16(in.data() + c);
+        dms_eng.get_output(dsp);
+        for (int i = 0; i < 3; i++)
+            dms_eng.wait_for_cpu[i] = false;
+        dms_eng.get_output(dup);
+        for (int i = 0; i < 3; i++) {
+            dms_eng.wait_for_cpu[i] = false;
+            dump(pre + ".dmstream.c64", dsp[i], per);
+            dump(pre + ".dmstream.u16", dup[i], per);
+        }
+    }
+
+    // independent streams with their own FFT sizes (msosm_gpu_batch_stream.h:14-49):
+    // stream 0 DM 10 at N 1024, stream 1 DM 20 at N 2048; each call consumes
+    // count*M[i] samples of stream i's own input (here both read the same
+    // sample array, each at its own position)
+    float sb[2] = {bw, bw}, sd[2] = {10.0f, 20.0f}, sf[2] = {f0, f0};
+    unsigned long sfp[2] = {1024, 2048};
+    MSOSM_GPU_BATCH_stream bst(sb, sd, sf, 2);
+    bst.initialize_uint16(sfp, 4);
+    meta << "bstream M " << bst.M[0] << " " << bst.M[1] << " fftpoint " << sfp[0] << " " << sfp[1] << "\n";
+    std::vector<std::vector<Complex>> bo{std::vector<Complex>((size_t)bst.count * bst.M[0]),
+                                         std::vector<Complex>((size_t)bst.count * bst.M[1])};
+    Complex *bop[2] = {bo[0].data(), bo[1].data()};
+    unsigned long pos[2] = {0, 0};
+    while (pos[1] + (unsigned long)bst.count * bst.M[1] <= L) {
+        uint16_pair *ip[2] = {in.data() + pos[0], in.data() + pos[1]};
+        bst.filter_block_uint16(ip);
+        bst.get_output(bop);
+        dump(pre + ".bstream0.c64", bop[0], bo[0].size());
+        dump(pre + ".bstream1.c64", bop[1], bo[1].size());
+        pos[0] += (unsigned long)bst.count * bst.M[0];
+        pos[1] += (unsigned long)bst.count * bst.M[1];
     }
 
     // single-segment OSM at DM 2 (M = 2 * nextpow2(Nd))

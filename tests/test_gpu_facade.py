@@ -65,3 +65,16 @@ def test_facade_program(device, tmp_path):
     np.testing.assert_array_equal(np.fromfile(pre + ".fold.f32", np.float32), f.folded())
     txt = open(pre + ".fold.txt").read().split()
     assert len(txt) == 100
+    # DM stream engine: per-trial hand-out of the same outputs as the
+    # concurrent engine, bit for bit, in both formats
+    ds = np.fromfile(pre + ".dmstream.c64", np.complex64).reshape(-1, 3, 8 * M)
+    np.testing.assert_array_equal(ds, conc)
+    np.testing.assert_array_equal(np.fromfile(pre + ".dmstream.u16", np.uint16), cu)
+    # independent streams with their own FFT sizes (M 512 and 1024)
+    assert "bstream M 512 1024 fftpoint 1024 2048" in meta
+    b0 = np.fromfile(pre + ".bstream0.c64", np.complex64)
+    b1 = np.fromfile(pre + ".bstream1.c64", np.complex64)
+    calls = b1.size // (4 * 1024)
+    assert calls >= 4 and b0.size == calls * 4 * 512
+    assert rel_l2(b0, ref(10.0)[:b0.size]) < 1e-5
+    assert rel_l2(b1, ref(20.0, M=1024)[:b1.size]) < 1e-5
