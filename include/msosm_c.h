@@ -316,6 +316,21 @@ int msosm_outring_push(msosm_outring *r, msosm_engine *eng, int format, int time
 int msosm_outring_pop(msosm_outring *r, int timeout_ms, const void **data, size_t *bytes, long long *seq);
 int msosm_outring_release(msosm_outring *r);
 
+/* The simulated test signal of the reference's drivers (SimulatedComplexSignal
+ * + AWGN, simulated_pulsar/simulated_complex_signal.cpp:12-236, AWGN.cpp:21-58):
+ * two unit impulses per period of Np = period*bw samples, dispersed at dm
+ * (spread over `range` periods; transforms of fftpoint points), repeated
+ * `repeat` times, optionally plus complex Gaussian noise at snr_db against
+ * the measured signal power.  signal: host, 2*repeat*Np floats (may be NULL);
+ * signal_u16: host, repeat*Np uint16 pairs (may be NULL).  Transforms on the
+ * GPU. */
+int msosm_refsim_info(float bw, float dm, float f0, float period, unsigned long *Np, unsigned long *Nd,
+                      unsigned long *range, unsigned long *fftpoint);
+int msosm_refsim_generate(float bw, float dm, float f0, float period, unsigned long repeat, int add_noise,
+                          float snr_db, float *signal, void *signal_u16, float *signal_power_db);
+/* The noise alone (host only): n complex samples as 2n floats. */
+int msosm_refsim_awgn(float snr_db, unsigned long n, float *out);
+
 /* Page-locked host memory for input and output buffers (replaces the
  * reference drivers' cudaHostRegister of their buffers, e.g.
  * test_processing_speed_msosm.cpp:182,196, which has no counterpart on this

@@ -92,6 +92,10 @@ SIGNATURES = {
     "msosm_k_discard_batch": (_i, [_vp, _vp, _i, _i, _i, _vp]),
     "msosm_k_fft_c2c": (_i, [_vp, _vp, _sz, _sz, _sz, _sz, _i, _vp]),
     "msosm_host_alloc": (_i, [_sz, ctypes.POINTER(_vp)]),
+    "msosm_refsim_info": (_i, [_f, _f, _f, _f, ctypes.POINTER(_ul), ctypes.POINTER(_ul), ctypes.POINTER(_ul),
+                               ctypes.POINTER(_ul)]),
+    "msosm_refsim_generate": (_i, [_f, _f, _f, _f, _ul, _i, _f, _vp, _vp, ctypes.POINTER(_f)]),
+    "msosm_refsim_awgn": (_i, [_f, _ul, _vp]),
     "msosm_outring_create": (_i, [_sz, _i, ctypes.POINTER(_vp)]),
     "msosm_outring_destroy": (_i, [_vp]),
     "msosm_outring_push": (_i, [_vp, _vp, _i, _i, ctypes.POINTER(_ll)]),
@@ -417,6 +421,30 @@ def fft_c2c(inp, out, n, batch, idist=None, odist=None, direction=-1, stream=Non
     idist = n if idist is None else idist
     odist = n if odist is None else odist
     check(lib().msosm_k_fft_c2c(_ptr(inp), _ptr(out), n, batch, idist, odist, direction, _stream_handle(stream)))
+
+
+def refsim_info(bw, dm, f0, period):
+    """(Np, Nd, range, fftpoint) of the reference drivers' simulated signal."""
+    v = [ctypes.c_ulong() for _ in range(4)]
+    check(lib().msosm_refsim_info(bw, dm, f0, period, *[ctypes.byref(x) for x in v]))
+    return tuple(int(x.value) for x in v)
+
+
+def refsim_generate(bw, dm, f0, period, repeat=1, add_noise=False, snr_db=0.0):
+    """(complex64 signal [repeat*Np], uint16 pairs [2*repeat*Np], signal power dB)."""
+    Np = refsim_info(bw, dm, f0, period)[0]
+    sig = np.empty(repeat * Np, np.complex64)
+    u16 = np.empty(2 * repeat * Np, np.uint16)
+    pw = ctypes.c_float()
+    check(lib().msosm_refsim_generate(bw, dm, f0, period, repeat, 1 if add_noise else 0, snr_db, sig.ctypes.data,
+                                      u16.ctypes.data, ctypes.byref(pw)))
+    return sig, u16, float(pw.value)
+
+
+def refsim_awgn(snr_db, n):
+    out = np.empty(n, np.complex64)
+    check(lib().msosm_refsim_awgn(snr_db, n, out.ctypes.data))
+    return out
 
 
 class OutRing:

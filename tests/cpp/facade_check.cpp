@@ -13,6 +13,7 @@
 #include "msosm/msosm_gpu_batch.h"
 #include "msosm/msosm_gpu_concurrent.h"
 #include "msosm/osm_gpu.h"
+#include "msosm/simulated_complex_signal.h"
 
 template <class T>
 static void dump(const std::string &path, const T *p, size_t n)
@@ -39,7 +40,7 @@ int main(int argc, char **argv)
         }
     }
     for (const char *s : {".batch.c64", ".conc.c64", ".conc.u16", ".osm.c64", ".loop.c64", ".fold.f32", ".inten.f32",
-                          ".dmstream.c64", ".dmstream.u16", ".bstream0.c64", ".bstream1.c64", ".nosync.f32"})
+                          ".dmstream.c64", ".dmstream.u16", ".bstream0.c64", ".bstream1.c64", ".sim.u16", ".sim.c64"})
         std::remove((pre + s).c_str());
     std::ofstream meta(pre + ".meta");
 
@@ -167,6 +168,16 @@ int main(int argc, char **argv)
     fold.write_to_file();
     dump(pre + ".fold.f32", fold.folded_data, fold.time_bin);
     meta << "fold P " << fold.period_samples << " time_bin " << fold.time_bin << "\n";
+    // the reference drivers' test signal (check_msosm.cpp:101-105): uint16,
+    // noise free, through the single-DM engine
+    {
+        SimulatedComplexSignal sim(bw, 10.0f, f0, 0.005f, "uint16");
+        sim.generate_pulsar_signal(4, false, 0, false);
+        meta << "sim Np " << sim.Np << " Nd " << sim.Nd << " range " << sim.range << " size " << sim.signal_size
+             << "\n";
+        dump(pre + ".sim.u16", sim.signal_u16, sim.signal_size);
+        dump(pre + ".sim.c64", &sim.signal[0][0], 2 * sim.signal_size);
+    }
     std::printf("facade_check ok\n");
     return 0;
 }

@@ -6,6 +6,7 @@ import subprocess
 import numpy as np
 import pytest
 
+import msosm_gpu_amd as ms
 import oracle
 
 pytestmark = pytest.mark.gpu
@@ -78,3 +79,9 @@ def test_facade_program(device, tmp_path):
     assert calls >= 4 and b0.size == calls * 4 * 512
     assert rel_l2(b0, ref(10.0)[:b0.size]) < 1e-5
     assert rel_l2(b1, ref(20.0, M=1024)[:b1.size]) < 1e-5
+    # the drivers' simulated signal through the facade class: the library's
+    # generator, four periods of 80,000 samples
+    assert "sim Np 80000 Nd 7743 range 1 size 320000" in meta
+    sig, u16, _ = ms.refsim_generate(bw, 10.0, f0, 0.005, repeat=4)
+    np.testing.assert_array_equal(np.fromfile(pre + ".sim.u16", np.uint16), u16)
+    np.testing.assert_array_equal(np.fromfile(pre + ".sim.c64", np.float32), sig.view(np.float32))
