@@ -196,6 +196,19 @@ __device__ __forceinline__ void fill_twiddles(float2 *twlo, float2 *twhi, int ti
     }
 }
 
+// Table of W_1024^j, j < 1024 (forward sign), for the inter-pass twiddles of
+// passes whose twiddle period NS*R divides 1024: one LDS read per twiddle in
+// place of digit products (used by the 2^14-point transforms).
+constexpr int kTw1k = 1024;
+__device__ __forceinline__ void fill_tw1k(float2 *tw, int tid, int nthreads)
+{
+    for (int i = tid; i < kTw1k; i += nthreads) {
+        double s, c;
+        sincospi(2.0 * (double)i / (double)kTw1k, &s, &c);
+        tw[i] = make_float2((float)c, (float)(-s));
+    }
+}
+
 template <int LOG2N, int E, bool INV>
 __device__ __forceinline__ float2 twiddle_lookup(const float2 *twlo, const float2 *twhi, int e)
 {
@@ -204,8 +217,9 @@ __device__ __forceinline__ float2 twiddle_lookup(const float2 *twlo, const float
     return INV ? make_float2(w.x, -w.y) : w;
 }
 
-template <int LOG2N, int E, bool INV, int P>
-__device__ __forceinline__ void fft_pass(float2 (&v)[E], float2 *lds, const float2 *twlo, const float2 *twhi, int t)
+template <int LOG2N, int E, bool INV, int P, bool T1K = false>
+__device__ __forceinline__ void fft_pass(float2 (&v)[E], float2 *lds, const float2 *twlo, const float2 *twhi, int t,
+                                         const float2 *tw1k = nullptr)
 {
     using S = FftShape<LOG2N, E>;
     constexpr int N = S::N, NT = S::NT, LOGE = S::LOGE, NPASS = S::NPASS;
@@ -231,7 +245,15 @@ __device__ __forceinline__ void fft_pass(float2 (&v)[E], float2 *lds, const floa
             u[r] = v[q + r * Q];
         const int j = t + q * NT;
         const int k = j & (NS - 1);
-        if constexpr (P > 0) {
+        if constexpr (P > 0 && T1K && NS * R <= kTw1k) {
+            // W_(NS R)^(k r) = W_1024^(k r 1024/(NS R)) from the table
+            constexpr int step = kTw1k / (NS * R);
+#pragma unroll
+            for (int r = 1; r < R; r++) {
+                const float2 w = tw1k[(k * r * step) & (kTw1k - 1)];
+                u[r] = cmul(u[r], INV ? make_float2(w.x, -w.y) : w);
+            }
+        } else if constexpr (P > 0) {
             // W^r for r = c + 4b + 16a from three lookups (W, W^4, W^16) and
             // products of base-4 digits: at most 4 roundings per twiddle.
             const int e1 = k << (LOG2N - LOGNS - LOGR);
@@ -273,15 +295,16 @@ __device__ __forceinline__ void fft_pass(float2 (&v)[E], float2 *lds, const floa
     }
     if constexpr (P < NPASS - 1) {
         __syncthreads();
-        fft_pass<LOG2N, E, INV, P + 1>(v, lds, twlo, twhi, t);
+        fft_pass<LOG2N, E, INV, P + 1, T1K>(v, lds, twlo, twhi, t, tw1k);
     }
 }
 
 // Full transform.  Every thread of the workgroup must call it (it contains
 // barriers).  lds must hold FftShape::NPAD float2 for this transform.  On
 // entry v[m] = x[t + m*NT]; on exit v[m] = X[t + m*NT] (unnormalised).
-template <int LOG2N, int E, bool INV>
-__device__ __forceinline__ void fft_block(float2 (&v)[E], float2 *lds, const float2 *twlo, const float2 *twhi, int t)
+template <int LOG2N, int E, bool INV, bool T1K = false>
+__device__ __forceinline__ void fft_block(float2 (&v)[E], float2 *lds, const float2 *twlo, const float2 *twhi, int t,
+                                          const float2 *tw1k = nullptr)
 {
     using S = FftShape<LOG2N, E>;
     if constexpr (S::NPASS == 1) {
@@ -295,7 +318,7 @@ __device__ __forceinline__ void fft_block(float2 (&v)[E], float2 *lds, const flo
             v[k] = o[k];
     } else {
         __syncthreads(); // LDS may still be read by a previous transform
-        fft_pass<LOG2N, E, INV, 0>(v, lds, twlo, twhi, t);
+        fft_pass<LOG2N, E, INV, 0, T1K>(v, lds, twlo, twhi, t, tw1k);
     }
 }
 

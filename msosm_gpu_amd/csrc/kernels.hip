@@ -531,11 +531,10 @@ __global__ __launch_bounds__((1 << LOG2N) / E) void k_inv_fused_nb(
     const char *ringB = reinterpret_cast<const char *>(ring);
     const T *hrow = reinterpret_cast<const T *>(H) + (long long)dm * N;
     const uint32_t *drow = dpk + (long long)dm * N / 2;
-    long long g = g0 + b0;
-    int slot = (int)(g % rx);
-    // blocks before the stream start read as zero; delays are < 65536, so
-    // a clamped 32-bit copy of g is enough for the comparison
-    int gc = (int)(g < 65536 ? g : 65536);
+    int slot = (int)((g0 + b0) % rx);
+    // A block before the stream start maps to slot rx + g - d, the slot of a
+    // block (> g0 + count - 1) that no call has written yet: the ring is
+    // zeroed at creation and reset, so those gathers read zero without a test.
     T raw[E];
     // byte offsets fit 32 bits (the ring is < 4 GiB): loads use a scalar base
     // plus one 32-bit VGPR offset
@@ -551,7 +550,7 @@ __global__ __launch_bounds__((1 << LOG2N) / E) void k_inv_fused_nb(
                 sl += sl < 0 ? rx : 0;
                 const uint32_t off =
                     ((uint32_t)sl * (uint32_t)N + (uint32_t)(t + m * NT)) * (uint32_t)sizeof(T);
-                raw[m] = gc >= d ? *reinterpret_cast<const T *>(ringB + off) : R::zero();
+                raw[m] = *reinterpret_cast<const T *>(ringB + off);
             }
         }
     };
@@ -567,12 +566,10 @@ __global__ __launch_bounds__((1 << LOG2N) / E) void k_inv_fused_nb(
             v[m] = cmul(R::f2(raw[m]), R::f2(hrow[t + m * NT]));
         if constexpr (PREFETCH) {
             if (i + 1 < nb) {
-                gc += gc < 65536 ? 1 : 0;
                 slot = slot + 1 == rx ? 0 : slot + 1;
                 gather();
             }
         } else {
-            gc += gc < 65536 ? 1 : 0;
             slot = slot + 1 == rx ? 0 : slot + 1;
         }
         fft_block<LOG2N, E, true>(v, smem, twlo, twhi, t);
