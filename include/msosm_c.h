@@ -172,8 +172,8 @@ int msosm_k_fft_c2c(const float *in, float *out, size_t n, size_t batch, size_t 
  * gpu_kernel.cu:160-197) over a shared spectra ring, as per-bin complex GEMMs
  * on the matrix cores: Y[dm][b][n] = H[dm][n] * ring[(g0 + b - d[dm][n]) mod rx][n]
  * (zero when g0 + b < d), for b < count, n < 2^log2n.  form: one of
- * MSOSM_FORM_HANKEL_*.  HANKEL_F16: ring and H are half2 (X/N, H/M), Y half2
- * holding the product / 2; BF16 forms: ring, H, Y complex64.  kmax: 1 + the
+ * MSOSM_FORM_HANKEL_*.  HANKEL_F16: ring, H and Y are half2 (the engine
+ * stores X/N and H/(2M)); BF16 forms: ring, H, Y complex64.  kmax: 1 + the
  * largest delay in the table. */
 int msosm_k_hankel_delay_gemm(const void *ring, const void *H, const uint16_t *delay, void *Y, int form,
                               int log2n, int count, int num_dms, long long g0, int rx, int kmax, void *stream);

@@ -90,7 +90,7 @@ struct msosm_engine {
     // shared buffers
     uint32_t *in_stage = nullptr;   // count*M packed input (host input path)
     float2 *H32 = nullptr;          // [ndm][2M]
-    __half2 *H16 = nullptr;         // [ndm][2M] = H/M (FUSED_F16)
+    __half2 *H16 = nullptr;         // [ndm][2M] = H/(2M) (FP16 forms)
     int32_t *delay32 = nullptr;     // [ndm][2M] (PLAIN)
     uint16_t *delay16 = nullptr;    // [ndm][2M] (FUSED)
     uint32_t *delay_pk = nullptr;   // [ndm][E/2][NT] packed pairs (FUSED, one transform per workgroup)
@@ -261,7 +261,10 @@ int build_filter_tables(msosm_engine *e)
         hipSuccess)
         return fail(err, "DFT_2M");
     if (e->H16) {
-        if ((err = msosm::launch_to_half(e->H16, e->H32, (size_t)ndm * N, 1.0f / (float)M, e->stream)) != hipSuccess)
+        // H/(2M) = H/N: with the ring's X/N the product is X H / N^2, whose
+        // unnormalised inverse transform is the output (the FP32 forms'
+        // IDFT(X H) / N^2) with no scale at the store
+        if ((err = msosm::launch_to_half(e->H16, e->H32, (size_t)ndm * N, 0.5f / (float)M, e->stream)) != hipSuccess)
             return fail(err, "H to FP16");
     }
     cleanup();
@@ -477,7 +480,7 @@ int filter_hankel(msosm_engine *e, const uint32_t *cur)
     }
     {
         StageScope st(e, MSOSM_STAGE_FFT_INV);
-        // FP16 products are (X/N)(H/M)/2, FP32 products X H: both give
+        // FP16 products are (X/N)(H/(2M)), FP32 products X H: both give
         // IDFT(X H) / N^2 after these scales
         const float scale = f16 ? 1.0f : 1.0f / ((float)N * (float)N);
         HIP_TRY(msosm::launch_inv_dense(e->ybuf, e->out, f16, e->log2n, (long long)ndm * count, scale, s));

@@ -19,8 +19,8 @@
 // Complex arithmetic uses the real embedding along k:
 //     Re Y = [Re A | -Im A] [Re B ; Im B],   Im Y = [Im A | Re A] [Re B ; Im B]
 // so one 32-deep MFMA step covers 16 complex k.  Operand forms:
-//   MODE_F16     FP16 ring X/N and FP16 H/M, v_mfma_f32_16x16x32_f16,
-//                Y stored FP16 as (X/N)(H/M)/2
+//   MODE_F16     FP16 ring X/N and FP16 H/(2M), v_mfma_f32_16x16x32_f16,
+//                Y stored FP16 as their product
 //   MODE_BF16    FP32 ring X and FP32 H rounded to BF16, one product,
 //                Y stored FP32 as X H
 //   MODE_BF16X3  as MODE_BF16 with split operands x = hi + lo (both BF16)
@@ -66,7 +66,7 @@ struct HankelMode<HANKEL_F16> {
     static constexpr bool SPLIT = false;
     __device__ static float2 load(const RingT *p) { return __half22float2(*p); }
     __device__ static f32x4 mma(OpV a, OpV b, f32x4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0); }
-    __device__ static YT out(float re, float im) { return __floats2half2_rn(0.5f * re, 0.5f * im); }
+    __device__ static YT out(float re, float im) { return __floats2half2_rn(re, im); }
 };
 
 template <>

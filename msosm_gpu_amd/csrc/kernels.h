@@ -34,8 +34,8 @@ bool fused_supported(int log2n);
 
 // Delay-line accumulate as per-bin complex GEMMs on MFMA (hankel.hip):
 // Y[dm][b][n] = H[dm][n] X_{g0+b-d(n)}[n] computed over the band of delays a
-// tile of 16 trials uses at bin n.  Modes: FP16 operands (ring X/N, H/M; Y
-// FP16 = product/2), BF16 operands from an FP32 ring and H (Y FP32 =
+// tile of 16 trials uses at bin n.  Modes: FP16 operands (ring X/N, H/(2M);
+// Y FP16 = product), BF16 operands from an FP32 ring and H (Y FP32 =
 // product), BF16 hi+lo split with three products.  kmax: largest delay + 1.
 constexpr int HANKEL_F16 = 0;
 constexpr int HANKEL_BF16 = 1;

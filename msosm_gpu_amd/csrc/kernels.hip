@@ -437,7 +437,8 @@ struct RingElem<false> {
 // Fused inverse: unit u = (dm, block b) with blocks of one DM consecutive.
 // Y[n] = X_{g-d(n)}[n] * H[n] (zero before the stream start), inverse FFT,
 // keep points [M, 2M), scale: FP32 ring/H -> 1/N^2 (the reference's two 1/N
-// factors, gpu_kernel.cu:169,251), FP16 ring (X/N) and H (H/M) -> 1/2.
+// factors, gpu_kernel.cu:169,251); FP16 ring (X/N) and H (H/(2M)) carry the
+// whole scale already.
 // This variant serves the small sizes (several transforms per workgroup).
 template <int LOG2N, bool F16>
 __global__ __launch_bounds__(fft_threads(LOG2N)) void k_inv_fused(
@@ -485,11 +486,11 @@ __global__ __launch_bounds__(fft_threads(LOG2N)) void k_inv_fused(
         v[m] = (valid && g >= d[m]) ? cmul(R::f2(xr[m]), R::f2(hr[m])) : make_float2(0.f, 0.f);
     fft_block<LOG2N, E, true>(v, smem + tl * S::NPAD, twlo, twhi, t);
     if (valid) {
-        const float scale = F16 ? 0.5f : 1.0f / ((float)N * (float)N);
+        const float scale = 1.0f / ((float)N * (float)N);
         float2 *dst = out + ((long long)dm * count + b) * M;
 #pragma unroll
         for (int m = E / 2; m < E; m++)
-            dst[t + m * S::NT - M] = cscale(v[m], scale);
+            dst[t + m * S::NT - M] = F16 ? v[m] : cscale(v[m], scale);
     }
 }
 
@@ -556,7 +557,7 @@ __global__ __launch_bounds__((1 << LOG2N) / E) void k_inv_fused_nb(
     };
     if constexpr (PREFETCH)
         gather();
-    const float scale = F16 ? 0.5f : 1.0f / ((float)N * (float)N);
+    const float scale = 1.0f / ((float)N * (float)N);
     for (int i = 0; i < nb; i++) {
         if constexpr (!PREFETCH)
             gather();
@@ -576,7 +577,7 @@ __global__ __launch_bounds__((1 << LOG2N) / E) void k_inv_fused_nb(
         float2 *dst = out + ((long long)dm * count + b0 + i) * M;
 #pragma unroll
         for (int m = E / 2; m < E; m++)
-            dst[t + m * NT - M] = cscale(v[m], scale);
+            dst[t + m * NT - M] = F16 ? v[m] : cscale(v[m], scale);
     }
 }
 

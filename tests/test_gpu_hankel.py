@@ -117,7 +117,7 @@ def test_gemm_kernel_vs_numpy(device, form):
     if f16:
         # exact FP16 products summed in FP32, then one FP16 rounding per component
         y = Y.cpu().numpy().astype(np.float64)
-        got = 2 * (y[..., 0] + 1j * y[..., 1])
+        got = y[..., 0] + 1j * y[..., 1]
         err = np.abs(got - exp) / np.maximum(np.abs(exp), 1e-30)
         print(f"form {form}: max relative element error {err[exp != 0].max():.2e}")
         assert err[exp != 0].max() < 2.0 ** -10
