@@ -98,6 +98,7 @@ def main():
     forms = [(f, form_ids[f]) for f in FORM_NAMES if f in wanted]
     plan_info = None
     fold = None
+    pcie = None
     hankel_flops = None
     for name, form in forms:
         plan = ms.Plan(bw, dms, f0, fftpoint=args.fftpoint, count=args.count, form=form)
@@ -134,6 +135,9 @@ def main():
         log(f"{name}: {ms_step:.2f} ms/step -> {value:.1f} {UNIT}; stages "
             + ", ".join(f"{k}={v['ms_per_launch']:.3f}ms" for k, v in results[name]["stages"].items()))
         if name == "mixed_f16":
+            pcie = host_input_rate(torch, eng, u, n, ncalls, len(dms), args.steps)
+            log(f"{name}, input from page-locked host memory: {pcie['ms_per_step']:.2f} ms/step -> "
+                f"{pcie['value']:.1f} {UNIT}; H2D copy {pcie['h2d_ms_per_call']:.3f} ms per call")
             fold = fold_bench(ms, torch, eng, plan, stream, 0.008 * bw, args.fold_bins)
             log(f"phase fold ({args.fold_bins} bins): {fold['ms_per_launch']:.3f} ms per launch, "
                 f"{fold['achieved']:.0f} GB/s")
@@ -191,6 +195,7 @@ def main():
                                      if "plain_f32" in results else None),
         "kernels": kernels,
         "hbm_copy_gbs": round(copy_gbs, 1),
+        "host_input_pcie": pcie,
         "rel_l2_vs_fp64": {k: v for k, v in checks.items()},
     }
     print(json.dumps(line), flush=True)
@@ -253,6 +258,34 @@ def copy_bandwidth(torch, dev, reps=5):
         best = max(best, 2 * src.numel() * 4 / (e0.elapsed_time(e1) / 1e3) / 1e9)
     del src, dst
     return best
+
+
+def host_input_rate(torch, eng, u, n, ncalls, ndm, steps):
+    """The same steps with the input handed over in page-locked host memory:
+    each call copies its count*M packed samples (4 B each) over PCIe before
+    its kernels run.  Wall-clock time, as the host waits for every copy."""
+    h_in = torch.from_numpy(u.view(np.int32)).pin_memory()
+
+    def step():
+        for c in range(ncalls):
+            eng.filter_u16(h_in[c * n:(c + 1) * n])
+
+    step()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    ms_step = (time.perf_counter() - t) * 1e3 / steps
+    eng.set_timing(True)
+    step()
+    timing = eng.get_timing()
+    eng.set_timing(False)
+    c = timing.get("h2d", (0.0, 0))
+    del h_in
+    return dict(value=round(ndm * ncalls * n / (ms_step / 1e3) / 1e6, 2), ms_per_step=round(ms_step, 3),
+                h2d_ms_per_call=round(c[0] / c[1], 4) if c[1] else None,
+                input="page-locked host memory, 4 B per sample copied host to device each call")
 
 
 def fold_bench(ms, torch, eng, plan, stream, period_samples, nbins, launches=20):

@@ -136,16 +136,19 @@ int msosm_engine_synchronize(msosm_engine *eng);
 int msosm_engine_reset(msosm_engine *eng);
 /* The HIP stream the engine launches on (void* = hipStream_t). */
 int msosm_engine_stream(msosm_engine *eng, void **hip_stream);
-/* Per-stage timing with HIP events on the engine stream (0 = off).
+/* Per-stage timing with HIP events on the engine stream (0 = off; the
+ * host-to-device copy of host input is timed on the engine's copy stream).
  * Stage ids: MSOSM_STAGE_*.  times_ms[i] = summed kernel time of stage i,
- * calls[i] = launches timed. */
+ * calls[i] = launches timed.  COPY is the device-side overlap-save history
+ * copy, H2D the copy of host input. */
 #define MSOSM_STAGE_UNPACK 0
 #define MSOSM_STAGE_FFT_FWD 1
 #define MSOSM_STAGE_MULTIPLY 2
 #define MSOSM_STAGE_FFT_INV 3
 #define MSOSM_STAGE_DISCARD 4
 #define MSOSM_STAGE_COPY 5
-#define MSOSM_NUM_STAGES 6
+#define MSOSM_STAGE_H2D 6
+#define MSOSM_NUM_STAGES 7
 int msosm_engine_set_timing(msosm_engine *eng, int on);
 int msosm_engine_get_timing(msosm_engine *eng, double *times_ms, long long *calls, int n);
 

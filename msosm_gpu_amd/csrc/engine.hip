@@ -88,7 +88,7 @@ struct msosm_engine {
     int log2n = 0;
     int form = 0;
     // shared buffers
-    uint32_t *in_stage = nullptr;   // count*M packed input (host input path)
+    uint32_t *in_stage = nullptr;   // 2 x count*M packed input (host input path, double-buffered)
     float2 *H32 = nullptr;          // [ndm][2M]
     __half2 *H16 = nullptr;         // [ndm][2M] = H/(2M) (FP16 forms)
     int32_t *delay32 = nullptr;     // [ndm][2M] (PLAIN)
@@ -113,7 +113,14 @@ struct msosm_engine {
     int kmax = 0;                   // largest delay + 1 over all trials
     long long g0 = 0;               // global index of the next block
     bool have_output = false;
-    hipEvent_t copy_done = nullptr; // host-input copy of the last call
+    // host input: copies run on copy_stream into alternate halves of in_stage;
+    // stage_free[i] is recorded on the engine stream after the kernels that
+    // read half i, and the next copy into that half waits for it
+    hipStream_t copy_stream = nullptr;
+    hipEvent_t copy_done = nullptr;
+    hipEvent_t stage_free[2] = {nullptr, nullptr};
+    bool stage_used[2] = {false, false};
+    int stage_slot = 0;
     // timing
     bool timing = false;
     std::vector<StageTimer> pending;
@@ -154,15 +161,16 @@ void harvest(msosm_engine *e)
 struct StageScope {
     msosm_engine *e;
     int stage;
+    hipStream_t s;
     hipEvent_t a = nullptr;
-    StageScope(msosm_engine *eng, int st) : e(eng), stage(st)
+    StageScope(msosm_engine *eng, int st, hipStream_t on = nullptr) : e(eng), stage(st), s(on ? on : eng->stream)
     {
         if (e->timing) {
             if (e->pending.size() > 4096)
                 harvest(e);
             a = pool_get(e);
             if (a)
-                hipEventRecord(a, e->stream);
+                hipEventRecord(a, s);
         }
     }
     ~StageScope()
@@ -170,7 +178,7 @@ struct StageScope {
         if (e->timing && a) {
             hipEvent_t b = pool_get(e);
             if (b) {
-                hipEventRecord(b, e->stream);
+                hipEventRecord(b, s);
                 e->pending.push_back({stage, a, b});
             } else {
                 e->pool.push_back(a);
@@ -188,8 +196,15 @@ void free_engine(msosm_engine *e)
     harvest(e);
     for (auto ev : e->pool)
         hipEventDestroy(ev);
+    if (e->copy_stream) {
+        hipStreamSynchronize(e->copy_stream);
+        hipStreamDestroy(e->copy_stream);
+    }
     if (e->copy_done)
         hipEventDestroy(e->copy_done);
+    for (hipEvent_t ev : e->stage_free)
+        if (ev)
+            hipEventDestroy(ev);
     void *bufs[] = {e->in_stage, e->H32, e->H16, e->delay32, e->delay16, e->delay_pk, e->out, e->out_u16, e->inbuf,
                     e->fftblk, e->ring, e->ifftblk, e->fft_scratch, e->tail, e->xring, e->ybuf};
     for (void *b : bufs)
@@ -333,7 +348,7 @@ int create_engine(const msosm::Plan &plan, void *stream, msosm_engine **out)
                                   (size_t)(n) * sizeof(*(ptr)), hipGetErrorString(a__)));                       \
     } while (0)
     const size_t out_elems = (size_t)ndm * count * M + 1;
-    ALLOC(e->in_stage, (size_t)count * M);
+    ALLOC(e->in_stage, 2 * (size_t)count * M);
     ALLOC(e->H32, (size_t)ndm * N);
     ALLOC(e->out, out_elems);
     ALLOC(e->out_u16, out_elems);
@@ -652,20 +667,31 @@ int msosm_engine_filter_u16(msosm_engine *e, const void *input, int where)
         return set_error(MSOSM_ERR_ARG, "filter_u16: NULL argument");
     const msosm::Plan &p = e->plan;
     const uint32_t *cur = (const uint32_t *)input;
+    const int slot = e->stage_slot;
     if (where == 0) {
-        // the caller may refill its host buffer as soon as this returns (the
-        // reference's blocking cudaMemcpy): wait for the copy, which in
-        // stream order also follows the previous call's kernels
-        {
-            StageScope st(e, MSOSM_STAGE_COPY);
-            HIP_TRY(hipMemcpyAsync(e->in_stage, input, sizeof(uint32_t) * (size_t)p.count * p.M,
-                                   hipMemcpyHostToDevice, e->stream));
-        }
-        if (!e->copy_done)
+        // The caller may refill its host buffer as soon as this returns (the
+        // reference's blocking cudaMemcpy), so wait for the copy.  It runs on
+        // the copy stream into the staging half that the call before the
+        // previous one read, so it overlaps the previous call's kernels
+        // instead of queueing behind them.
+        if (!e->copy_stream) {
+            HIP_TRY(hipStreamCreateWithFlags(&e->copy_stream, hipStreamNonBlocking));
             HIP_TRY(hipEventCreateWithFlags(&e->copy_done, hipEventDisableTiming));
-        HIP_TRY(hipEventRecord(e->copy_done, e->stream));
+            HIP_TRY(hipEventCreateWithFlags(&e->stage_free[0], hipEventDisableTiming));
+            HIP_TRY(hipEventCreateWithFlags(&e->stage_free[1], hipEventDisableTiming));
+        }
+        uint32_t *dst = e->in_stage + (size_t)slot * p.count * p.M;
+        if (e->stage_used[slot])
+            HIP_TRY(hipStreamWaitEvent(e->copy_stream, e->stage_free[slot], 0));
+        {
+            StageScope st(e, MSOSM_STAGE_H2D, e->copy_stream);
+            HIP_TRY(hipMemcpyAsync(dst, input, sizeof(uint32_t) * (size_t)p.count * p.M, hipMemcpyHostToDevice,
+                                   e->copy_stream));
+        }
+        HIP_TRY(hipEventRecord(e->copy_done, e->copy_stream));
         HIP_TRY(hipEventSynchronize(e->copy_done));
-        cur = e->in_stage;
+        HIP_TRY(hipStreamWaitEvent(e->stream, e->copy_done, 0));
+        cur = dst;
     } else if (where != 1) {
         return set_error(MSOSM_ERR_ARG, "filter_u16: where must be 0 (host) or 1 (device)");
     }
@@ -674,6 +700,11 @@ int msosm_engine_filter_u16(msosm_engine *e, const void *input, int where)
                                                 : filter_fused(e, cur);
     if (rc == MSOSM_OK)
         e->have_output = true;
+    if (where == 0) {
+        HIP_TRY(hipEventRecord(e->stage_free[slot], e->stream));
+        e->stage_used[slot] = true;
+        e->stage_slot = slot ^ 1;
+    }
     return rc;
 }
 

@@ -144,6 +144,31 @@ def test_host_and_device_input_agree(device):
     np.testing.assert_array_equal(a, b)
 
 
+@pytest.mark.parametrize("form", [ms.FORM_PLAIN_F32, ms.FORM_FUSED_F16])
+def test_host_input_refilled_buffer_pipelined(device, form):
+    # One host buffer refilled before every call and no output read in
+    # between: the copies alternate staging halves and overlap the previous
+    # call's kernels.  The last output depends on the previous K+1 calls'
+    # input (K = 16 with count 1), so a copy that overwrote a half still being
+    # read, or a call that read stale input, changes it.
+    bw, dm, f0 = 16e6, 10.0, 1392e6
+    p = ms.Plan(bw, dm, f0, fftpoint=1024, count=1, form=form)
+    assert p.K_max == 16
+    ncalls = 40
+    u = make_input(ncalls * 512, 31).view(np.uint32)
+    ref = run_engine(p, u.view(np.uint16), device=device)[0].reshape(ncalls, 512)
+    eng = ms.Engine(p)
+    buf = np.empty(512, np.uint32)
+    for c in range(ncalls):
+        buf[:] = u[c * 512:(c + 1) * 512]
+        eng.filter_u16(buf)
+        if c == 25:
+            np.testing.assert_array_equal(eng.output_c64()[0, 0], ref[c])
+    last = eng.output_c64()[0, 0]
+    eng.close()
+    np.testing.assert_array_equal(last, ref[-1])
+
+
 def test_forms_agree(device):
     bw, f0 = 8e6, 1396e6
     dms = [50.0, 300.0]
@@ -213,4 +238,5 @@ def test_timing_reports_stages(device):
     t = eng.get_timing()
     eng.close()
     assert t["fft_fwd"][1] == 3 and t["fft_inv"][1] == 3 and t["multiply"][1] == 3
+    assert t["h2d"][1] == 3  # host input: one copy per call, timed on the copy stream
     assert all(v[0] >= 0 for v in t.values())
