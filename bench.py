@@ -54,7 +54,7 @@ def parse():
                          "device copy (the counter calibration), no timing")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-check", action="store_true", help="skip the FP64 spot check")
-    ap.add_argument("--cpu-log2-samples", type=int, default=25)
+    ap.add_argument("--cpu-log2-samples", type=int, default=23)
     return ap.parse_args()
 
 
@@ -386,9 +386,11 @@ def spot_check(ms, eng, plan, d_in, u, bw, f0, dms, n):
     return dict(blocks=[b0, b0 + nb], per_dm=errs, max_rel_l2=max(errs.values()))
 
 
-def cpu_baseline(bw, f0, dms, M, u, log2_samples):
+def cpu_baseline(bw, f0, dms, M, u, log2_samples, reps=5, log2_single=20):
     """The oracle's FP32 MS-OSM restatement (OpenMP over DM x block) on the
-    host cores, on the first 2^log2_samples samples for all DM trials."""
+    host cores: median of `reps` runs over the first 2^log2_samples samples
+    for all DM trials with every core the process may use, after a warm-up;
+    and one single-thread run over the first 2^log2_single samples."""
     import oracle
     ncores = len(os.sched_getaffinity(0))
     Lc = 1 << log2_samples
@@ -396,12 +398,21 @@ def cpu_baseline(bw, f0, dms, M, u, log2_samples):
     H = np.stack([oracle.dedisp_params(bw, float(dm), f0, M) for dm in dms]).astype(np.complex64)
     us = np.ascontiguousarray(u[:2 * Lc])
     oracle.cpu_msosm_f32(us[:2 * 4 * M], M, delays[:2], H[:2], nthreads=ncores)  # warm-up
+    times = []
+    for _ in range(reps):
+        t = time.perf_counter()
+        oracle.cpu_msosm_f32(us, M, delays, H, nthreads=ncores)
+        times.append(time.perf_counter() - t)
+    dt = float(np.median(times))
+    L1 = 1 << log2_single
     t = time.perf_counter()
-    oracle.cpu_msosm_f32(us, M, delays, H, nthreads=ncores)
-    dt = time.perf_counter() - t
+    oracle.cpu_msosm_f32(np.ascontiguousarray(u[:2 * L1]), M, delays, H, nthreads=1)
+    dt1 = time.perf_counter() - t
     return dict(value=round(len(dms) * Lc / dt / 1e6, 3), unit=UNIT, cores=ncores, kind="port",
-                sample="first 2^%d samples x %d DM trials, FP32 MS-OSM (oracle restatement), %.1f s"
-                       % (log2_samples, len(dms), dt))
+                sample="first 2^%d samples x %d DM trials, FP32 MS-OSM (oracle restatement), median of %d runs "
+                       "(%.1f s each)" % (log2_samples, len(dms), reps, dt),
+                single_thread=dict(value=round(len(dms) * L1 / dt1 / 1e6, 3), cores=1,
+                                   sample="first 2^%d samples x %d DM trials, %.1f s" % (log2_single, len(dms), dt1)))
 
 
 if __name__ == "__main__":

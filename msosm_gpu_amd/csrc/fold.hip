@@ -82,12 +82,20 @@ __global__ void k_fold_bins(float *__restrict__ folded, const float *__restrict_
 // Each workgroup folds one contiguous chunk of its trial's samples, split
 // into one contiguous piece per wave, so it touches only the bins its chunk
 // spans and adds only those (non-zero) bins to the global profile.  A wave
-// reads 8 groups of 64 consecutive samples per step (8 loads in flight).
-// While every sample of a group falls into the bin the wave is accumulating
-// (the usual case when a bin spans many samples), each lane just adds its
-// value to a register; when the bin changes the wave sums its registers with
-// lane shuffles and one lane adds the total to the LDS histogram.  Groups that
-// straddle bins fall back to one LDS atomic per lane.
+// reads kFoldUnroll groups of 64*V consecutive samples per step (V = 2: one
+// 16-byte load of two samples per lane, when the rows are 16-byte aligned).
+//
+// The bin of a sample only changes at bin edges, and within one period the
+// FP64 phase, hence the bin, is non-decreasing in the sample index (every
+// step of the phase evaluation is monotonic; ph - floor(ph) is exact).  So a
+// group whose first and last samples fall into the same bin of the same
+// period lies wholly in that bin: per step, lanes 2u and 2u+1 evaluate the
+// phase of group u's first and last sample, and only groups that straddle a
+// bin edge evaluate it per sample.  A wave adds the samples of whole-bin
+// groups to a register while the bin stays the same and sums the registers
+// with lane shuffles when it changes; a straddling group is split by bin
+// with ballots (a few partial sums), and lanes left after that add to the
+// LDS histogram one by one.
 __device__ __forceinline__ float wave_sum(float x)
 {
 #pragma unroll
@@ -98,13 +106,39 @@ __device__ __forceinline__ float wave_sum(float x)
 
 constexpr int kFoldThreads = 256;
 constexpr int kFoldUnroll = 8;
+constexpr int kFoldPeel = 3; // partial sums per straddling group before per-lane atomics
 
-__global__ __launch_bounds__(kFoldThreads) void k_fold_phase(const float2 *__restrict__ a,
+__device__ __forceinline__ float pw(float2 v) { return v.x * v.x + v.y * v.y; }
+
+// V consecutive complex64 samples of one lane: one 8- or 16-byte load
+template <int V>
+struct Lanes;
+template <>
+struct Lanes<1> {
+    float2 v;
+    __device__ __forceinline__ static Lanes load(const float2 *p) { return {*p}; }
+    __device__ __forceinline__ float power(int) const { return pw(v); }
+};
+template <>
+struct Lanes<2> {
+    float4 v;
+    __device__ __forceinline__ static Lanes load(const float2 *p) { return {*reinterpret_cast<const float4 *>(p)}; }
+    __device__ __forceinline__ float power(int s) const
+    {
+        return s ? pw(make_float2(v.z, v.w)) : pw(make_float2(v.x, v.y));
+    }
+};
+
+// One polarisation: registers for 8 waves per SIMD (no spills); two: as needed.
+template <int V, bool TWO>
+__global__ __launch_bounds__(kFoldThreads, TWO ? 1 : 8) void k_fold_phase(const float2 *__restrict__ a,
                                                              const float2 *__restrict__ b, long long n,
                                                              long long stride, long long chunk, long long n0,
                                                              double inv_period, int nbins, int power,
                                                              float *__restrict__ prof, float *__restrict__ hits)
 {
+    static_assert(V == 1 || V == 2, "one or two samples per lane and load");
+    static_assert(2 * kFoldUnroll <= 64, "two lanes per group evaluate its end phases");
     extern __shared__ float hist[]; // nbins values (+ nbins hit counts for trial 0)
     const int dm = blockIdx.y;
     const bool count_hits = (hits != nullptr) && dm == 0;
@@ -112,79 +146,163 @@ __global__ __launch_bounds__(kFoldThreads) void k_fold_phase(const float2 *__res
     for (int i = threadIdx.x; i < nh; i += blockDim.x)
         hist[i] = 0.0f;
     __syncthreads();
+    // this wave's parking area for the values of straddling groups
+    float *stage = hist + nbins * (hits != nullptr ? 2 : 1) + (threadIdx.x >> 6) * (kFoldUnroll * 64 * V);
     const float2 *ad = a + (long long)dm * stride;
     const float2 *bd = b ? b + (long long)dm * stride : nullptr;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     constexpr int waves = kFoldThreads / 64;
-    constexpr int group_step = 64 * kFoldUnroll;
+    constexpr int span = 64 * V; // samples per wave and load
+    constexpr int group_step = span * kFoldUnroll;
     // chunk is a multiple of waves * group_step: equal contiguous pieces
     const long long piece = chunk / waves;
     const long long w0 = (long long)blockIdx.x * chunk + wave * piece;
     const long long w1 = min(w0 + piece, n);
-    float acc = 0.f, cnt = 0.f;
-    int cur = -1; // wave-uniform bin being accumulated in registers
+    // bin of global sample index i; *whole = floor of the phase (period number)
+    auto phase_bin = [&](long long i, double *whole) {
+        double ph = (double)i * inv_period;
+        const double f = floor(ph);
+        ph -= f;
+        if (whole)
+            *whole = f;
+        const int bin = (int)(ph * (double)nbins);
+        return bin < nbins ? bin : nbins - 1;
+    };
+    float acc = 0.f;  // per-lane sum of the samples of bin cur
+    float ucnt = 0.f; // samples added to acc (wave-uniform)
+    int cur = -1;     // wave-uniform bin being accumulated in registers
     auto flush = [&]() {
         if (cur >= 0) {
             const float sum = wave_sum(acc);
-            const float c = count_hits ? wave_sum(cnt) : 0.f;
             if (lane == 0) {
                 atomicAdd(&hist[cur], sum);
                 if (count_hits)
-                    atomicAdd(&hist[nbins + cur], c);
+                    atomicAdd(&hist[nbins + cur], ucnt);
             }
         }
         acc = 0.f;
-        cnt = 0.f;
+        ucnt = 0.f;
         cur = -1;
     };
-    // Index arithmetic per step is uniform (scalar); per lane only the offset
-    // u*64 + lane varies.  The phase is frac((n0 + i) / P) in FP64.
-    for (long long base = w0; base < w1; base += group_step) { // wave-uniform bound
-        const long long rem = w1 - base; // samples left for this wave, uniform
-        const float2 *ga = ad + base;
-        const float2 *gb = bd ? bd + base : nullptr;
-        float p[kFoldUnroll];
+    // fold one step of samples base .. base+min(rem, group_step) whose values
+    // are in p (uniform base and rem)
+    auto process = [&](float (&p)[kFoldUnroll][V], long long base, long long rem) {
+        if (!power) {
+#pragma unroll
+            for (int u = 0; u < kFoldUnroll; u++)
+#pragma unroll
+                for (int s = 0; s < V; s++)
+                    p[u][s] = sqrtf(p[u][s]);
+        }
+        // end phases of the groups: lane 2u -> first sample of group u, lane
+        // 2u+1 -> its last sample (clamped to the piece); other lanes unused
+        int ebin, eper;
+        {
+            const int gu = lane >> 1;
+            long long idx = (lane & 1) ? min((long long)(gu + 1) * span, rem) - 1 : (long long)gu * span;
+            idx = (lane < 2 * kFoldUnroll && idx >= 0 && idx < rem) ? idx : 0;
+            double whole;
+            ebin = phase_bin(n0 + base + idx, &whole);
+            eper = (int)(long long)whole; // period number, low 32 bits
+        }
+        unsigned smask = 0; // groups that straddle a bin edge (uniform)
 #pragma unroll
         for (int u = 0; u < kFoldUnroll; u++) {
-            const int off = u * 64 + lane;
-            p[u] = 0.f;
-            if (off < rem) {
-                const float2 v = ga[off];
-                p[u] = v.x * v.x + v.y * v.y;
-                if (gb) {
-                    const float2 w = gb[off];
-                    p[u] += w.x * w.x + w.y * w.y;
+            if (u * span >= rem) // uniform: nothing left in this group or after
+                break;
+            const int bf = __builtin_amdgcn_readlane(ebin, 2 * u), bl = __builtin_amdgcn_readlane(ebin, 2 * u + 1);
+            const int pf = __builtin_amdgcn_readlane(eper, 2 * u), pl = __builtin_amdgcn_readlane(eper, 2 * u + 1);
+            if (bf == bl && (pf == pl || nbins == 1)) { // uniform: the whole group is in bin bf
+                if (bf != cur) {
+                    flush();
+                    cur = bf;
+                }
+#pragma unroll
+                for (int s = 0; s < V; s++)
+                    acc += p[u][s];
+                ucnt += (float)min((long long)span, rem - u * span);
+                continue;
+            }
+            // straddles a bin edge: park the values for the per-sample pass
+#pragma unroll
+            for (int s = 0; s < V; s++)
+                stage[u * span + lane * V + s] = p[u][s];
+            smask |= 1u << u;
+        }
+        // per-sample pass over the straddling groups (a runtime loop: only the
+        // values of one group are live)
+        while (smask) { // uniform
+            const int u = __builtin_ctz(smask);
+            smask &= smask - 1;
+#pragma unroll
+            for (int s = 0; s < V; s++) {
+                const int off = u * span + lane * V + s;
+                bool left = off < rem;
+                const int bin = left ? phase_bin(n0 + base + off, nullptr) : 0;
+                const float v = stage[off]; // written by this lane
+                for (int it = 0; it < kFoldPeel; it++) {
+                    const unsigned long long todo = __ballot(left);
+                    if (todo == 0) // uniform
+                        break;
+                    const int bsel = __shfl(bin, __ffsll((long long)todo) - 1);
+                    const bool mine = left && bin == bsel;
+                    const float sum = wave_sum(mine ? v : 0.f);
+                    const unsigned long long mm = __ballot(mine);
+                    if (lane == 0) {
+                        atomicAdd(&hist[bsel], sum);
+                        if (count_hits)
+                            atomicAdd(&hist[nbins + bsel], (float)__popcll(mm));
+                    }
+                    left = left && !mine;
+                }
+                if (left) {
+                    atomicAdd(&hist[bin], v);
+                    if (count_hits)
+                        atomicAdd(&hist[nbins + bin], 1.0f);
                 }
             }
         }
+    };
+    // whole steps: every load of a step issued before any is used
+    const long long nfull = w1 > w0 ? (w1 - w0) / group_step : 0;
+    for (long long k = 0; k < nfull; k++) { // wave-uniform bound
+        const long long base = w0 + k * group_step;
+        Lanes<V> ra[kFoldUnroll], rb[TWO ? kFoldUnroll : 1];
 #pragma unroll
         for (int u = 0; u < kFoldUnroll; u++) {
-            if (u * 64 >= rem) // uniform: nothing left in this group or after
-                break;
-            const bool valid = u * 64 + lane < rem;
-            float v = p[u];
-            if (!power)
-                v = sqrtf(v);
-            double ph = (double)(n0 + base + u * 64 + lane) * inv_period;
-            ph -= floor(ph);
-            int bin = (int)(ph * (double)nbins);
-            bin = bin < nbins ? bin : nbins - 1;
-            bin = valid ? bin : -1;
-            const int b0 = __builtin_amdgcn_readfirstlane(bin); // lane 0 is valid here
-            const bool uni = __all(!valid || bin == b0);       // uniform
-            if (!uni || b0 != cur) {                            // uniform
-                flush();
-                cur = uni ? b0 : -1;
-            }
-            if (uni) {
-                acc += v;
-                cnt += valid ? 1.0f : 0.0f;
-            } else if (valid) {
-                atomicAdd(&hist[bin], v);
-                if (count_hits)
-                    atomicAdd(&hist[nbins + bin], 1.0f);
-            }
+            ra[u] = Lanes<V>::load(ad + base + u * span + lane * V);
+            if constexpr (TWO)
+                rb[u] = Lanes<V>::load(bd + base + u * span + lane * V);
         }
+        float p[kFoldUnroll][V];
+#pragma unroll
+        for (int u = 0; u < kFoldUnroll; u++)
+#pragma unroll
+            for (int s = 0; s < V; s++) {
+                p[u][s] = ra[u].power(s);
+                if constexpr (TWO)
+                    p[u][s] += rb[u].power(s);
+            }
+        process(p, base, group_step);
+    }
+    // the last, partial step of a row: one sample at a time
+    const long long tb = w0 + nfull * group_step;
+    if (tb < w1) {
+        const long long rem = w1 - tb;
+        float p[kFoldUnroll][V];
+#pragma unroll
+        for (int u = 0; u < kFoldUnroll; u++)
+#pragma unroll
+            for (int s = 0; s < V; s++) {
+                const int off = u * span + lane * V + s;
+                p[u][s] = 0.f;
+                if (off < rem) {
+                    p[u][s] = pw(ad[tb + off]);
+                    if constexpr (TWO)
+                        p[u][s] += pw(bd[tb + off]);
+                }
+            }
+        process(p, tb, rem);
     }
     flush();
     __syncthreads();
@@ -554,24 +672,40 @@ int msosm_fold_phase(const void *a, const void *b, size_t n, int ndm, size_t str
                                    "stride >= n)");
     if (ndm > 65535)
         return ferr(MSOSM_ERR_ARG, "fold_phase: at most 65535 DM trials per launch");
-    // contiguous chunks of at least 16384 samples (a multiple of one step of
+    // two samples per lane and load when every row starts 16-byte aligned
+    const bool wide = ((uintptr_t)a % 16 == 0) && (!b || (uintptr_t)b % 16 == 0) && (ndm == 1 || stride % 2 == 0);
+    // contiguous chunks of at least 8192 samples (a multiple of one step of
     // the 4 waves), at most 4096 chunks per trial; MSOSM_FOLD_CHUNK overrides
     // the minimum (tuning runs)
     static const long long min_chunk = [] {
         const char *e = getenv("MSOSM_FOLD_CHUNK");
-        long long c = e ? atoll(e) : 16384;
-        return c >= 1024 ? c : 16384;
+        long long c = e ? atoll(e) : 8192;
+        return c >= 1024 ? c : 8192;
     }();
-    const long long step = (long long)(kFoldThreads / 64) * 64 * kFoldUnroll; // one group step of every wave
+    const long long step = (long long)(kFoldThreads / 64) * 64 * (wide ? 2 : 1) * kFoldUnroll;
     long long chunk = min_chunk;
     while ((long long)(n + chunk - 1) / chunk > 4096)
         chunk *= 2;
     chunk = (chunk + step - 1) / step * step;
     const long long gx = ((long long)n + chunk - 1) / chunk;
-    size_t lds = sizeof(float) * (size_t)nbins * (hits ? 2 : 1);
-    hipLaunchKernelGGL(k_fold_phase, dim3((unsigned)gx, ndm), dim3(kFoldThreads), lds, (hipStream_t)stream,
-                       (const float2 *)a, (const float2 *)b, (long long)n, (long long)stride, chunk, n0,
-                       1.0 / period_samples, nbins, power, prof, hits);
+    // histogram (+ hit counts) and one parking area per wave (a step's values)
+    const size_t lds = sizeof(float) * ((size_t)nbins * (hits ? 2 : 1) +
+                                        (size_t)(kFoldThreads / 64) * kFoldUnroll * 64 * (wide ? 2 : 1));
+    auto launch = [&](auto kern) {
+        if (lds > 65536) {
+            const hipError_t e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                     (int)lds);
+            if (e != hipSuccess)
+                return e;
+        }
+        hipLaunchKernelGGL(kern, dim3((unsigned)gx, ndm), dim3(kFoldThreads), lds, (hipStream_t)stream,
+                           (const float2 *)a, (const float2 *)b, (long long)n, (long long)stride, chunk, n0,
+                           1.0 / period_samples, nbins, power, prof, hits);
+        return hipSuccess;
+    };
+    const hipError_t le = wide ? (b ? launch(k_fold_phase<2, true>) : launch(k_fold_phase<2, false>))
+                               : (b ? launch(k_fold_phase<1, true>) : launch(k_fold_phase<1, false>));
+    FHIP(le);
     FHIP(hipGetLastError());
     return MSOSM_OK;
 }

@@ -78,18 +78,24 @@ def test_intensity_then_fold_end_to_end(device):
 
 
 @pytest.mark.parametrize("P", [1234.567, 987654.321])
-def test_phase_fold_lds_histogram(device, P):
+@pytest.mark.parametrize("n,stride,first", [(100000, 100352, 0), (99999, 100352, 0), (99999, 100001, 0),
+                                            (99998, 100352, 1)])
+def test_phase_fold_lds_histogram(device, P, n, stride, first):
     # FP64 phase per sample, LDS histogram per workgroup: bin assignment
     # (hit counts) exact, sums to float tolerance, for 3 trials at once; the
     # long period puts whole waves into one bin (wave-sum path), the short
-    # one splits most waves over several bins
+    # one splits most waves over several bins.  Rows 16-byte aligned (two
+    # samples per lane and load, also with an odd length), an odd stride and
+    # rows starting at an odd sample (one sample per lane and load).
     import torch
     rng = np.random.default_rng(9)
-    ndm, n, stride, nbins, n0 = 3, 100000, 100352, 64, 5000
+    ndm, nbins, n0 = 3, 64, 5000
     a = (rng.standard_normal((ndm, stride)) + 1j * rng.standard_normal((ndm, stride))).astype(np.complex64)
     b = (rng.standard_normal((ndm, stride)) + 1j * rng.standard_normal((ndm, stride))).astype(np.complex64)
-    ad = torch.from_numpy(a).to(device)
-    bd = torch.from_numpy(b).to(device)
+    ad = torch.from_numpy(a).to(device).reshape(-1)[first:]
+    bd = torch.from_numpy(b).to(device).reshape(-1)[first:]
+    a = np.stack([a.reshape(-1)[first + d * stride:first + d * stride + n] for d in range(ndm)])
+    b = np.stack([b.reshape(-1)[first + d * stride:first + d * stride + n] for d in range(ndm)])
     for power in (True, False):
         prof = torch.zeros((ndm, nbins), dtype=torch.float32, device=device)
         hits = torch.zeros(nbins, dtype=torch.float32, device=device)
@@ -104,3 +110,23 @@ def test_phase_fold_lds_histogram(device, P):
             v = np.sqrt(v)
         exp = np.stack([np.bincount(bins, weights=v[d], minlength=nbins) for d in range(ndm)])
         assert np.max(np.abs(prof.cpu().numpy() - exp) / np.maximum(exp, 1e-30)) < 1e-5
+
+
+def test_phase_fold_max_bins(device):
+    # 8192 bins (the most one launch takes): the histogram and hit counts
+    # plus the per-wave staging areas exceed 64 KiB of LDS per workgroup
+    import torch
+    rng = np.random.default_rng(10)
+    n, nbins, P = 300000, 8192, 250000.5
+    a = (rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(np.complex64)
+    ad = torch.from_numpy(a).to(device)
+    prof = torch.zeros((1, nbins), dtype=torch.float32, device=device)
+    hits = torch.zeros(nbins, dtype=torch.float32, device=device)
+    ms.fold_phase(ad, n, P, nbins, prof, hits)
+    torch.cuda.synchronize()
+    ph = np.arange(n, dtype=np.float64) * (1.0 / P)
+    ph -= np.floor(ph)
+    bins = np.minimum((ph * nbins).astype(np.int64), nbins - 1)
+    np.testing.assert_array_equal(hits.cpu().numpy(), np.bincount(bins, minlength=nbins).astype(np.float32))
+    exp = np.bincount(bins, weights=np.abs(a.astype(np.complex128)) ** 2, minlength=nbins)
+    assert np.max(np.abs(prof.cpu().numpy()[0] - exp) / np.maximum(exp, 1e-30)) < 1e-5
