@@ -196,15 +196,20 @@ __device__ __forceinline__ void fill_twiddles(float2 *twlo, float2 *twhi, int ti
     }
 }
 
-// Table of W_1024^j, j < 1024 (forward sign), for the inter-pass twiddles of
-// passes whose twiddle period NS*R divides 1024: one LDS read per twiddle in
-// place of digit products (used by the 2^14-point transforms).
-constexpr int kTw1k = 1024;
-__device__ __forceinline__ void fill_tw1k(float2 *tw, int tid, int nthreads)
+// Row table of the inter-pass twiddles of a pass with NS = 32 and radix 32
+// (the middle pass of the 2^14-point transform with 32 points per thread):
+// row k holds W_1024^(k r) for r < 32 (forward sign), rows padded to 33
+// entries.  A thread's k is fixed, so its 31 reads are one base address plus
+// immediate offsets, and the 32 rows a wave reads start in distinct banks.
+constexpr int kTwRowN = 32;      // rows (NS) and columns (R)
+constexpr int kTwRowStride = 33; // padded row length
+constexpr int kTwRowElems = kTwRowN * kTwRowStride;
+__device__ __forceinline__ void fill_twrows(float2 *tw, int tid, int nthreads)
 {
-    for (int i = tid; i < kTw1k; i += nthreads) {
+    for (int i = tid; i < kTwRowElems; i += nthreads) {
+        const int k = i / kTwRowStride, r = i - k * kTwRowStride;
         double s, c;
-        sincospi(2.0 * (double)i / (double)kTw1k, &s, &c);
+        sincospi(2.0 * (double)((k * r) & (kTwRowN * kTwRowN - 1)) / (double)(kTwRowN * kTwRowN), &s, &c);
         tw[i] = make_float2((float)c, (float)(-s));
     }
 }
@@ -217,9 +222,9 @@ __device__ __forceinline__ float2 twiddle_lookup(const float2 *twlo, const float
     return INV ? make_float2(w.x, -w.y) : w;
 }
 
-template <int LOG2N, int E, bool INV, int P, bool T1K = false>
+template <int LOG2N, int E, bool INV, int P, bool ROWS = false>
 __device__ __forceinline__ void fft_pass(float2 (&v)[E], float2 *lds, const float2 *twlo, const float2 *twhi, int t,
-                                         const float2 *tw1k = nullptr)
+                                         const float2 *twrows = nullptr)
 {
     using S = FftShape<LOG2N, E>;
     constexpr int N = S::N, NT = S::NT, LOGE = S::LOGE, NPASS = S::NPASS;
@@ -245,12 +250,12 @@ __device__ __forceinline__ void fft_pass(float2 (&v)[E], float2 *lds, const floa
             u[r] = v[q + r * Q];
         const int j = t + q * NT;
         const int k = j & (NS - 1);
-        if constexpr (P > 0 && T1K && NS * R <= kTw1k) {
-            // W_(NS R)^(k r) = W_1024^(k r 1024/(NS R)) from the table
-            constexpr int step = kTw1k / (NS * R);
+        if constexpr (P > 0 && ROWS && NS == kTwRowN && R == kTwRowN) {
+            // W_(NS R)^(k r) = W_1024^(k r) from row k of the table
+            const float2 *row = twrows + k * kTwRowStride;
 #pragma unroll
             for (int r = 1; r < R; r++) {
-                const float2 w = tw1k[(k * r * step) & (kTw1k - 1)];
+                const float2 w = row[r];
                 u[r] = cmul(u[r], INV ? make_float2(w.x, -w.y) : w);
             }
         } else if constexpr (P > 0) {
@@ -295,16 +300,16 @@ __device__ __forceinline__ void fft_pass(float2 (&v)[E], float2 *lds, const floa
     }
     if constexpr (P < NPASS - 1) {
         __syncthreads();
-        fft_pass<LOG2N, E, INV, P + 1, T1K>(v, lds, twlo, twhi, t, tw1k);
+        fft_pass<LOG2N, E, INV, P + 1, ROWS>(v, lds, twlo, twhi, t, twrows);
     }
 }
 
 // Full transform.  Every thread of the workgroup must call it (it contains
 // barriers).  lds must hold FftShape::NPAD float2 for this transform.  On
 // entry v[m] = x[t + m*NT]; on exit v[m] = X[t + m*NT] (unnormalised).
-template <int LOG2N, int E, bool INV, bool T1K = false>
+template <int LOG2N, int E, bool INV, bool ROWS = false>
 __device__ __forceinline__ void fft_block(float2 (&v)[E], float2 *lds, const float2 *twlo, const float2 *twhi, int t,
-                                          const float2 *tw1k = nullptr)
+                                          const float2 *twrows = nullptr)
 {
     using S = FftShape<LOG2N, E>;
     if constexpr (S::NPASS == 1) {
@@ -318,7 +323,7 @@ __device__ __forceinline__ void fft_block(float2 (&v)[E], float2 *lds, const flo
             v[k] = o[k];
     } else {
         __syncthreads(); // LDS may still be read by a previous transform
-        fft_pass<LOG2N, E, INV, 0, T1K>(v, lds, twlo, twhi, t, tw1k);
+        fft_pass<LOG2N, E, INV, 0, ROWS>(v, lds, twlo, twhi, t, twrows);
     }
 }
 

@@ -426,12 +426,23 @@ struct RingElem<true> {
     using T = __half2;
     __device__ static float2 f2(T x) { return __half22float2(x); }
     __device__ static T zero() { return __floats2half2_rn(0.f, 0.f); }
+    // x * h in FP32 from FP16 operands: four mixed-precision FMAs that read
+    // the halves directly (v_fma_mix_f32), no separate conversions; the
+    // partial products xi*hi and xi*hr are rounded once each, as in an FP32
+    // multiply.
+    __device__ __forceinline__ static float2 mul(T x, T h)
+    {
+        const float xr = __low2float(x), xi = __high2float(x), hr = __low2float(h), hi = __high2float(h);
+        const float t1 = __builtin_fmaf(xi, hi, 0.0f), t2 = __builtin_fmaf(xi, hr, 0.0f);
+        return make_float2(__builtin_fmaf(xr, hr, -t1), __builtin_fmaf(xr, hi, t2));
+    }
 };
 template <>
 struct RingElem<false> {
     using T = float2;
     __device__ static float2 f2(T x) { return x; }
     __device__ static T zero() { return make_float2(0.f, 0.f); }
+    __device__ __forceinline__ static float2 mul(T x, T h) { return cmul(x, h); }
 };
 
 // Fused inverse: unit u = (dm, block b) with blocks of one DM consecutive.
@@ -483,7 +494,7 @@ __global__ __launch_bounds__(fft_threads(LOG2N)) void k_inv_fused(
     float2 v[E];
 #pragma unroll
     for (int m = 0; m < E; m++)
-        v[m] = (valid && g >= d[m]) ? cmul(R::f2(xr[m]), R::f2(hr[m])) : make_float2(0.f, 0.f);
+        v[m] = (valid && g >= d[m]) ? R::mul(xr[m], hr[m]) : make_float2(0.f, 0.f);
     fft_block<LOG2N, E, true>(v, smem + tl * S::NPAD, twlo, twhi, t);
     if (valid) {
         const float scale = 1.0f / ((float)N * (float)N);
@@ -515,9 +526,14 @@ __global__ __launch_bounds__((1 << LOG2N) / E) void k_inv_fused_nb(
     constexpr int NT = S::NT;
     using R = RingElem<F16>;
     using T = typename R::T;
-    __shared__ float2 smem[S::NPAD + S::TWLO + S::TWHI];
-    float2 *twlo = smem + S::NPAD, *twhi = twlo + S::TWLO;
+    // the middle pass of 2^14 points with 32 per thread reads its twiddles
+    // from a row table (one LDS read per twiddle instead of digit products)
+    constexpr bool ROWS = (LOG2N == 14 && E == 32);
+    __shared__ float2 smem[S::NPAD + S::TWLO + S::TWHI + (ROWS ? kTwRowElems : 0)];
+    float2 *twlo = smem + S::NPAD, *twhi = twlo + S::TWLO, *twrows = twhi + S::TWHI;
     fill_twiddles<LOG2N, E>(twlo, twhi, threadIdx.x, blockDim.x);
+    if constexpr (ROWS)
+        fill_twrows(twrows, threadIdx.x, blockDim.x);
     const int t = threadIdx.x;
     const int chunks = (count + NB - 1) / NB;
     const int G = gridDim.x;
@@ -564,7 +580,7 @@ __global__ __launch_bounds__((1 << LOG2N) / E) void k_inv_fused_nb(
         float2 v[E];
 #pragma unroll
         for (int m = 0; m < E; m++)
-            v[m] = cmul(R::f2(raw[m]), R::f2(hrow[t + m * NT]));
+            v[m] = R::mul(raw[m], hrow[t + m * NT]);
         if constexpr (PREFETCH) {
             if (i + 1 < nb) {
                 slot = slot + 1 == rx ? 0 : slot + 1;
@@ -573,7 +589,7 @@ __global__ __launch_bounds__((1 << LOG2N) / E) void k_inv_fused_nb(
         } else {
             slot = slot + 1 == rx ? 0 : slot + 1;
         }
-        fft_block<LOG2N, E, true>(v, smem, twlo, twhi, t);
+        fft_block<LOG2N, E, true, ROWS>(v, smem, twlo, twhi, t, twrows);
         float2 *dst = out + ((long long)dm * count + b0 + i) * M;
 #pragma unroll
         for (int m = E / 2; m < E; m++)
