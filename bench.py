@@ -33,7 +33,7 @@ METRIC = "DM-trial·Msamples/s on 1 MI355X, mixed vs FP32 plain at ≤1e-3 rel e
 UNIT = "DM-trial·Msamples/s"
 HBM_PEAK_GBS = 8000.0
 MFMA_F16_PEAK_TFLOPS = 2500.0
-FORM_NAMES = ("plain_f32", "mixed_f16", "hankel_f16", "hankel_bf16x3")
+FORM_NAMES = ("plain_f32", "mixed_f16", "mixed_f16_mfma", "hankel_f16", "hankel_bf16x3")
 
 
 def parse():
@@ -88,7 +88,8 @@ def main():
     stream = torch.cuda.current_stream()
     results = {}
     checks = {}
-    form_ids = {"plain_f32": ms.FORM_PLAIN_F32, "mixed_f16": ms.FORM_FUSED_F16, "hankel_f16": ms.FORM_HANKEL_F16,
+    form_ids = {"plain_f32": ms.FORM_PLAIN_F32, "mixed_f16": ms.FORM_FUSED_F16,
+                "mixed_f16_mfma": ms.FORM_FUSED_F16_MFMA, "hankel_f16": ms.FORM_HANKEL_F16,
                 "hankel_bf16x3": ms.FORM_HANKEL_BF16X3}
     wanted = [f for f in args.forms.split(",") if f]
     if "mixed_f16" not in wanted:
@@ -346,6 +347,10 @@ def kernel_rooflines(results, plan_info, ndm, fold, hankel_flops):
         put("k_fft_c2c<14,inverse> (plain)", ndm * count * N * 16, sp["fft_inv"]["ms_per_launch"])
         # per (trial, bin): read H, write the ring slot (8 B each); the spectrum is shared by all trials
         put("k_multiply_delay (plain)", ndm * count * N * 16, sp["multiply"]["ms_per_launch"])
+    if "mixed_f16_mfma" in results:
+        # same algorithmic bytes as the vector-ALU fused inverse
+        put("k_inv_mfma<14> (matrix-core DFTs)", ndm * count * M * 16,
+            results["mixed_f16_mfma"]["stages"]["fft_inv"]["ms_per_launch"])
     for name, mode, terms in (("hankel_f16", "fp16", 1), ("hankel_bf16x3", "bf16x3", 3)):
         if name not in results or hankel_flops is None:
             continue

@@ -54,6 +54,10 @@ extern "C" {
 #define MSOSM_FORM_HANKEL_F16 3    /* FP16 operands, FP32 accumulate         */
 #define MSOSM_FORM_HANKEL_BF16 4   /* BF16 operands (single), FP32 accumulate */
 #define MSOSM_FORM_HANKEL_BF16X3 5 /* BF16 hi+lo split operands, 3 products  */
+/* As FUSED_F16 with the inverse FFT's DFT stages on the matrix cores (FP16 */
+/* data entering each radix pass, FP32 accumulate and twiddles); FFT size   */
+/* 16384 only:                                                              */
+#define MSOSM_FORM_FUSED_F16_MFMA 6
 
 typedef struct msosm_plan_desc {
     int num_dms;             /* DM trials sharing one input stream (>=1)      */

@@ -22,6 +22,7 @@ METHOD_MSOSM, METHOD_OSM = 0, 1
 CHIRP_REF_F32W, CHIRP_EXACT_F64 = 0, 1
 FORM_PLAIN_F32, FORM_FUSED_F32, FORM_FUSED_F16 = 0, 1, 2
 FORM_HANKEL_F16, FORM_HANKEL_BF16, FORM_HANKEL_BF16X3 = 3, 4, 5
+FORM_FUSED_F16_MFMA = 6
 STAGES = ("unpack", "fft_fwd", "multiply", "fft_inv", "discard", "copy", "h2d")
 
 _lib = None

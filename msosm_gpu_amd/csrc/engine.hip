@@ -24,6 +24,7 @@
 #include "common.h"
 #include "engine_internal.h"
 #include "kernels.h"
+#include "mfma_fft.h"
 #include "msosm_c.h"
 #include "plan.h"
 
@@ -91,6 +92,9 @@ struct msosm_engine {
     uint32_t *in_stage = nullptr;   // 2 x count*M packed input (host input path, double-buffered)
     float2 *H32 = nullptr;          // [ndm][2M]
     __half2 *H16 = nullptr;         // [ndm][2M] = H/(2M) (FP16 forms)
+    __half2 *Hp = nullptr;          // H16 in the matrix-core gather order (FUSED_F16_MFMA)
+    uint32_t *dpk_mfma = nullptr;   // delays in that order, packed in pairs (FUSED_F16_MFMA)
+    uint8_t *mfma_tab = nullptr;    // DFT tiles and twiddles of the matrix-core inverse
     int32_t *delay32 = nullptr;     // [ndm][2M] (PLAIN)
     uint16_t *delay16 = nullptr;    // [ndm][2M] (FUSED)
     uint32_t *delay_pk = nullptr;   // [ndm][E/2][NT] packed pairs (FUSED, one transform per workgroup)
@@ -205,7 +209,8 @@ void free_engine(msosm_engine *e)
     for (hipEvent_t ev : e->stage_free)
         if (ev)
             hipEventDestroy(ev);
-    void *bufs[] = {e->in_stage, e->H32, e->H16, e->delay32, e->delay16, e->delay_pk, e->out, e->out_u16, e->inbuf,
+    void *bufs[] = {e->in_stage, e->H32,  e->H16,     e->delay32,  e->delay16, e->delay_pk, e->Hp,
+                    e->dpk_mfma, e->mfma_tab, e->out, e->out_u16, e->inbuf,
                     e->fftblk, e->ring, e->ifftblk, e->fft_scratch, e->tail, e->xring, e->ybuf};
     for (void *b : bufs)
         if (b)
@@ -286,7 +291,10 @@ int build_filter_tables(msosm_engine *e)
     return rc;
 }
 
-bool form_f16_ring(int form) { return form == MSOSM_FORM_FUSED_F16 || form == MSOSM_FORM_HANKEL_F16; }
+bool form_f16_ring(int form)
+{
+    return form == MSOSM_FORM_FUSED_F16 || form == MSOSM_FORM_FUSED_F16_MFMA || form == MSOSM_FORM_HANKEL_F16;
+}
 bool form_hankel(int form)
 {
     return form == MSOSM_FORM_HANKEL_F16 || form == MSOSM_FORM_HANKEL_BF16 || form == MSOSM_FORM_HANKEL_BF16X3;
@@ -313,6 +321,9 @@ int create_engine(const msosm::Plan &plan, void *stream, msosm_engine **out)
     if (e->form != MSOSM_FORM_PLAIN_F32 && !msosm::fused_supported(e->log2n))
         return bail(set_error(MSOSM_ERR_UNSUPPORTED, "fused forms support FFT sizes 4..16384 (got %lu); use "
                                                      "MSOSM_FORM_PLAIN_F32", N));
+    if (e->form == MSOSM_FORM_FUSED_F16_MFMA && e->log2n != msosm::mfft::kLog2N)
+        return bail(set_error(MSOSM_ERR_UNSUPPORTED, "MSOSM_FORM_FUSED_F16_MFMA needs an FFT size of %d (plan has %lu)",
+                              msosm::mfft::kN, N));
     if (form_hankel(e->form)) {
         e->hankel_mode = hankel_mode_of(e->form);
         int km = 0;
@@ -404,7 +415,24 @@ int create_engine(const msosm::Plan &plan, void *stream, msosm_engine **out)
         (void)hipFree(tmp);
         if (herr != hipSuccess)
             return bail(set_error(MSOSM_ERR_HIP, "delay table upload: %s", hipGetErrorString(herr)));
-        if (!form_hankel(e->form) && msosm::fused_uses_packed_delay(e->log2n)) {
+        if (e->form == MSOSM_FORM_FUSED_F16_MFMA) {
+            // delays in the matrix-core gather order, pairs of positions p0
+            // and p0 + 64 (elements i and i + 1 of a lane's K-tile)
+            std::vector<uint32_t> pk((size_t)ndm * N / 2);
+            for (int d = 0; d < ndm; d++) {
+                const int32_t *row = p.delay.data() + (size_t)d * N;
+                uint32_t *dst = pk.data() + (size_t)d * N / 2;
+                for (int pp = 0; pp < (int)(N / 2); pp++) {
+                    const int p0 = ((pp >> 7) * 4 + 2 * ((pp >> 6) & 1)) * 64 + (pp & 63);
+                    dst[pp] = (uint32_t)row[msosm::mfft::ring_bin(p0)] |
+                              ((uint32_t)row[msosm::mfft::ring_bin(p0 + 64)] << 16);
+                }
+            }
+            ALLOC(e->dpk_mfma, pk.size());
+            HIP_TRY(hipMemcpy(e->dpk_mfma, pk.data(), sizeof(uint32_t) * pk.size(), hipMemcpyHostToDevice));
+            ALLOC(e->Hp, (size_t)ndm * N);
+            ALLOC(e->mfma_tab, msosm::inv_mfma_tables_bytes());
+        } else if (!form_hankel(e->form) && msosm::fused_uses_packed_delay(e->log2n)) {
             const int NT = msosm::fused_delay_threads(e->log2n);
             const int E = (int)(N / NT);
             std::vector<uint32_t> pk((size_t)ndm * N / 2);
@@ -426,6 +454,10 @@ int create_engine(const msosm::Plan &plan, void *stream, msosm_engine **out)
     int rc = build_filter_tables(e);
     if (rc != MSOSM_OK)
         return bail(rc);
+    if (e->form == MSOSM_FORM_FUSED_F16_MFMA) {
+        HIP_TRY(msosm::launch_inv_mfma_filter(e->Hp, e->H16, ndm, e->stream));
+        HIP_TRY(msosm::launch_inv_mfma_tables(e->mfma_tab, e->stream));
+    }
     HIP_TRY(hipStreamSynchronize(e->stream));
     *out = e;
     return MSOSM_OK;
@@ -482,7 +514,7 @@ int filter_hankel(msosm_engine *e, const uint32_t *cur)
     hipStream_t s = e->stream;
     {
         StageScope st(e, MSOSM_STAGE_FFT_FWD);
-        HIP_TRY(msosm::launch_fwd_u16(cur, e->tail, e->xring, f16, e->log2n, count, e->g0, e->rx, s));
+        HIP_TRY(msosm::launch_fwd_u16(cur, e->tail, e->xring, f16, e->log2n, count, e->g0, e->rx, false, s));
     }
     {
         StageScope st(e, MSOSM_STAGE_COPY);
@@ -509,11 +541,12 @@ int filter_fused(msosm_engine *e, const uint32_t *cur)
     const msosm::Plan &p = e->plan;
     const unsigned long M = p.M;
     const int ndm = p.num_dms, count = p.count;
-    const bool f16 = e->form == MSOSM_FORM_FUSED_F16;
+    const bool mfma = e->form == MSOSM_FORM_FUSED_F16_MFMA;
+    const bool f16 = e->form == MSOSM_FORM_FUSED_F16 || mfma;
     hipStream_t s = e->stream;
     {
         StageScope st(e, MSOSM_STAGE_FFT_FWD);
-        HIP_TRY(msosm::launch_fwd_u16(cur, e->tail, e->xring, f16, e->log2n, count, e->g0, e->rx, s));
+        HIP_TRY(msosm::launch_fwd_u16(cur, e->tail, e->xring, f16, e->log2n, count, e->g0, e->rx, mfma, s));
     }
     {
         StageScope st(e, MSOSM_STAGE_COPY);
@@ -521,8 +554,12 @@ int filter_fused(msosm_engine *e, const uint32_t *cur)
     }
     {
         StageScope st(e, MSOSM_STAGE_FFT_INV);
-        HIP_TRY(msosm::launch_inv_fused(e->xring, f16 ? (const void *)e->H16 : (const void *)e->H32, e->delay16,
-                                        e->delay_pk, e->out, f16, e->log2n, count, ndm, e->g0, e->rx, s));
+        if (mfma)
+            HIP_TRY(msosm::launch_inv_mfma(static_cast<const __half2 *>(e->xring), e->Hp, e->dpk_mfma, e->mfma_tab,
+                                           e->out, count, ndm, e->g0, e->rx, s));
+        else
+            HIP_TRY(msosm::launch_inv_fused(e->xring, f16 ? (const void *)e->H16 : (const void *)e->H32, e->delay16,
+                                            e->delay_pk, e->out, f16, e->log2n, count, ndm, e->g0, e->rx, s));
     }
     e->g0 += count;
     return MSOSM_OK;

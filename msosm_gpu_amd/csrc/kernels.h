@@ -18,9 +18,10 @@ size_t fft_c2c_scratch_elems(int log2n, long long batch);
 
 // Fused forward FFT of count overlapped windows read straight from packed
 // uint16 input (cur) and the previous call's last M samples (tail), written
-// to slots (g0 + w) mod rx of a spectra ring (FP32, or FP16 scaled by 1/N).
+// to slots (g0 + w) mod rx of a spectra ring (FP32, or FP16 scaled by 1/N);
+// mfma_order (FP16, 2^14 points only): bin n at position mfft::ring_pos(n).
 hipError_t launch_fwd_u16(const uint32_t *cur, const uint32_t *tail, void *ring, bool f16, int log2n, int count,
-                          long long g0, int rx, hipStream_t s);
+                          long long g0, int rx, bool mfma_order, hipStream_t s);
 // Fused inverse: per (DM, block) gather X_{g-d(n)}[n] * H[n] from the ring,
 // inverse FFT, keep the last M samples, scale; out layout [dm][count][M].
 // dpk: for sizes with one transform per workgroup (fused_uses_packed_delay),
@@ -31,6 +32,20 @@ hipError_t launch_inv_fused(const void *ring, const void *H, const uint16_t *del
 int fused_delay_threads(int log2n);
 bool fused_uses_packed_delay(int log2n);
 bool fused_supported(int log2n);
+
+// Fused inverse with the DFT stages on the matrix cores (invmfma.hip), 2^14
+// points: ring, Hp and dpk in the gather order of mfft::ring_pos (Hp from
+// launch_inv_mfma_filter; dpk[dm][pp] = d(p0) | d(p0 + 64) << 16 with
+// p0 = ((pp >> 7) * 4 + 2 ((pp >> 6) & 1)) * 64 + (pp & 63)); tables: a
+// device buffer of inv_mfma_tables_bytes() filled once by
+// launch_inv_mfma_tables.
+size_t inv_mfma_tables_bytes();
+hipError_t launch_inv_mfma_tables(void *tables, hipStream_t s);
+hipError_t launch_inv_mfma(const __half2 *ring, const __half2 *Hp, const uint32_t *dpk, const void *tables,
+                           float2 *out, int count, int ndm, long long g0, int rx, hipStream_t s);
+// Filter rows for launch_inv_mfma from the FP16 rows H/(2M): dst[r][ring_pos(n)]
+// = src[r][n] / 64 (FP16 headroom; the kernel's last DFT carries the 64)
+hipError_t launch_inv_mfma_filter(__half2 *dst, const __half2 *src, long long rows, hipStream_t s);
 
 // Delay-line accumulate as per-bin complex GEMMs on MFMA (hankel.hip):
 // Y[dm][b][n] = H[dm][n] X_{g0+b-d(n)}[n] computed over the band of delays a

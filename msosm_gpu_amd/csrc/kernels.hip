@@ -9,6 +9,7 @@
 //    discard + scale in one kernel;
 //  * the FP64 chirp generator for the filter tables.
 #include "kernels.h"
+#include "mfma_fft.h"
 
 #include <hip/hip_fp16.h>
 
@@ -376,7 +377,7 @@ __global__ __launch_bounds__(cols_threads(LOG2N)) void k_fft_passB(
 // from the previous call's tail.  The spectrum goes to ring slot
 // (g0 + w) mod rx, either FP32 or FP16 scaled by 1/N (keeps |X| in range:
 // |x| <= 32768*sqrt(2) bounds |X|/N by 46341 < 65504).
-template <int LOG2N, bool F16>
+template <int LOG2N, bool F16, bool MFMA_ORDER = false>
 __global__ __launch_bounds__(fft_threads(LOG2N)) void k_fwd_u16(
     const uint32_t *__restrict__ cur, const uint32_t *__restrict__ tail, void *__restrict__ ring, int count,
     long long g0, int rx)
@@ -408,8 +409,12 @@ __global__ __launch_bounds__(fft_threads(LOG2N)) void k_fwd_u16(
             __half2 *dst = reinterpret_cast<__half2 *>(ring) + slot * N;
             constexpr float invN = 1.0f / (float)N;
 #pragma unroll
-            for (int m = 0; m < E; m++)
-                dst[t + m * S::NT] = __floats2half2_rn(v[m].x * invN, v[m].y * invN);
+            for (int m = 0; m < E; m++) {
+                // MFMA_ORDER: bin n at position mfft::ring_pos(n), the gather
+                // order of the matrix-core inverse (2^14 points)
+                const int n = t + m * S::NT;
+                dst[MFMA_ORDER ? mfft::ring_pos(n) : n] = __floats2half2_rn(v[m].x * invN, v[m].y * invN);
+            }
         } else {
             float2 *dst = reinterpret_cast<float2 *>(ring) + slot * N;
 #pragma unroll
@@ -738,11 +743,23 @@ bool fused_supported(int log2n) { return log2n >= 2 && log2n <= kMaxLog2SinglePa
 
 template <int LOG2N>
 static hipError_t fwd_u16(const uint32_t *cur, const uint32_t *tail, void *ring, bool f16, int count, long long g0,
-                          int rx, hipStream_t s)
+                          int rx, bool mfma_order, hipStream_t s)
 {
     constexpr int E = fft_e(LOG2N);
     constexpr int threads = FftShape<LOG2N, E>::NT * fft_tpb(LOG2N);
     const int grid = (count + fft_tpb(LOG2N) - 1) / fft_tpb(LOG2N);
+    if constexpr (LOG2N == mfft::kLog2N) {
+        if (mfma_order) {
+            if (!f16)
+                return hipErrorInvalidValue;
+            hipLaunchKernelGGL((k_fwd_u16<LOG2N, true, true>), dim3(grid), dim3(threads), 0, s, cur, tail, ring,
+                               count, g0, rx);
+            MSOSM_LAUNCH_CHECK();
+            return hipSuccess;
+        }
+    } else if (mfma_order) {
+        return hipErrorInvalidValue;
+    }
     if (f16)
         hipLaunchKernelGGL((k_fwd_u16<LOG2N, true>), dim3(grid), dim3(threads), 0, s, cur, tail, ring, count, g0,
                            rx);
@@ -848,9 +865,9 @@ static hipError_t inv_fused(const void *ring, const void *H, const uint16_t *del
     }
 
 hipError_t launch_fwd_u16(const uint32_t *cur, const uint32_t *tail, void *ring, bool f16, int log2n, int count,
-                          long long g0, int rx, hipStream_t s)
+                          long long g0, int rx, bool mfma_order, hipStream_t s)
 {
-#define CALL_F(L) fwd_u16<L>(cur, tail, ring, f16, count, g0, rx, s)
+#define CALL_F(L) fwd_u16<L>(cur, tail, ring, f16, count, g0, rx, mfma_order, s)
     MSOSM_DISPATCH_2_14(log2n, CALL_F)
 #undef CALL_F
 }

@@ -142,7 +142,7 @@ int build_plan(const msosm_plan_desc &d, Plan &p, std::string &msg)
         msg = "plan: unknown chirp mode";
         return MSOSM_ERR_ARG;
     }
-    if (d.form < MSOSM_FORM_PLAIN_F32 || d.form > MSOSM_FORM_HANKEL_BF16X3) {
+    if (d.form < MSOSM_FORM_PLAIN_F32 || d.form > MSOSM_FORM_FUSED_F16_MFMA) {
         msg = "plan: unknown execution form";
         return MSOSM_ERR_ARG;
     }

@@ -8,7 +8,7 @@ trial, and graph replay against eager execution.
   - single trial at production size: 2^28 samples, 128 MHz, DM 100, FFT 2^16
     (K = 152), reference-structured FP32 form
   - many trials: 2^26 samples, 64 MHz, 256 trials on [0, 100], FFT 2^14,
-    FP16 and FP32 fused forms
+    FP16 (vector ALU and matrix-core inverse) and FP32 fused forms
   - 24 independent streams of different bandwidth (8-64 MHz), length
     (2^20-2^25 samples), trial count (1-64, DM 0-300) and autotuned FFT size,
     each phase-folded into 512 bins, in one hipGraph
@@ -77,7 +77,7 @@ def test_many_trials_full_stream(device):
     L = 1 << 26
     d_in = device_input(L, 0x5EED0003, device)
     outs = {}
-    for form in (ms.FORM_FUSED_F16, ms.FORM_FUSED_F32):
+    for form in (ms.FORM_FUSED_F16, ms.FORM_FUSED_F16_MFMA, ms.FORM_FUSED_F32):
         p = ms.Plan(bw, dms, f0, fftpoint=16384, count=64, form=form)
         n = p.count * p.M
         eng = ms.Engine(p, stream=torch.cuda.current_stream())
@@ -93,6 +93,11 @@ def test_many_trials_full_stream(device):
     per_dm = (torch.linalg.vector_norm((a - b).reshape(len(dms), -1), dim=1) /
               torch.linalg.vector_norm(b.reshape(len(dms), -1), dim=1)).cpu().numpy()
     print(f"FP16 vs FP32 fused, all 256 trials: max rel L2 {per_dm.max():.2e}", flush=True)
+    assert per_dm.max() < 1e-3
+    c = outs[ms.FORM_FUSED_F16_MFMA]
+    per_dm = (torch.linalg.vector_norm((c - b).reshape(len(dms), -1), dim=1) /
+              torch.linalg.vector_norm(b.reshape(len(dms), -1), dim=1)).cpu().numpy()
+    print(f"FP16 matrix-core vs FP32 fused, all 256 trials: max rel L2 {per_dm.max():.2e}", flush=True)
     assert per_dm.max() < 1e-3
     M, K = 8192, p.K_max
     b_last = L // M - p.count
