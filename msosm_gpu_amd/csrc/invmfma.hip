@@ -94,7 +94,7 @@ __global__ void k_mfma_tables(Tables *t)
         const double scale = tile < 8 ? 1.0 : (double)kHeadroom;
         f16x8 a;
         for (int jj = 0; jj < 8; jj++)
-            a[jj] = (_Float16)(scale * dft_entry(R, 8 * mt + (c >> 1), c & 1, 16 * kt + 4 * q + (jj >> 1), jj & 1));
+            a[jj] = (_Float16)(scale * dft_entry(R, 8 * mt + (c >> 1), c & 1, 16 * kt + 4 * (jj >> 1) + q, jj & 1));
         t->tiles[tile][lane] = a;
     }
     if (i < kTwRowElems) {
@@ -240,7 +240,7 @@ __global__ __launch_bounds__(kThreads, 2) void k_inv_mfma(const __half2 *__restr
             const int jp = 32 * c + 4 * wave + gi;
 #pragma unroll
             for (int kt = 0; kt < 2; kt++)
-                b[gi][kt] = *reinterpret_cast<const f16x8 *>(&a1[a1_addr(jp, 16 * kt + 4 * q)]);
+                b[gi][kt] = *reinterpret_cast<const f16x8 *>(&a1[a1_addr(jp, 16 * kt + q)]);
         }
         __syncthreads(); // every wave has read A1 before A2 (the same memory) is written
         __builtin_amdgcn_sched_barrier(0);
@@ -268,7 +268,7 @@ __global__ __launch_bounds__(kThreads, 2) void k_inv_mfma(const __half2 *__restr
         f16x8 bb[8];
 #pragma unroll
         for (int gi = 0; gi < 8; gi++)
-            bb[gi] = *reinterpret_cast<const f16x8 *>(&a2[a2_addr(16 * (8 * wave + gi) + c, 4 * q)]);
+            bb[gi] = *reinterpret_cast<const f16x8 *>(&a2[a2_addr(16 * (8 * wave + gi) + c, q)]);
 #pragma unroll
         for (int gi = 0; gi < 8; gi++) {
             const int jpp = 16 * (8 * wave + gi) + c;
