@@ -324,6 +324,22 @@ int create_engine(const msosm::Plan &plan, void *stream, msosm_engine **out)
     if (e->form == MSOSM_FORM_FUSED_F16_MFMA && e->log2n != msosm::mfft::kLog2N)
         return bail(set_error(MSOSM_ERR_UNSUPPORTED, "MSOSM_FORM_FUSED_F16_MFMA needs an FFT size of %d (plan has %lu)",
                               msosm::mfft::kN, N));
+    if (e->form != MSOSM_FORM_PLAIN_F32) {
+        // the shared-ring forms keep delays in 16 bits and address the spectra
+        // ring with 32-bit byte offsets
+        if (p.K_max > 65536)
+            return bail(set_error(MSOSM_ERR_UNSUPPORTED,
+                                  "delays of up to %d blocks do not fit the 16-bit delay tables of the shared-ring "
+                                  "forms; use a larger FFT size or MSOSM_FORM_PLAIN_F32",
+                                  p.K_max - 1));
+        const size_t ring_bytes = (size_t)(p.K_max - 1 + p.count) * N *
+                                  (form_f16_ring(e->form) ? sizeof(__half2) : sizeof(float2));
+        if (ring_bytes >= ((size_t)1 << 32))
+            return bail(set_error(MSOSM_ERR_UNSUPPORTED,
+                                  "the spectra ring (%zu bytes for %d blocks of history and %d per call) must stay "
+                                  "below 4 GiB: use fewer blocks per call or MSOSM_FORM_PLAIN_F32",
+                                  ring_bytes, p.K_max - 1, p.count));
+    }
     if (form_hankel(e->form)) {
         e->hankel_mode = hankel_mode_of(e->form);
         int km = 0;

@@ -51,3 +51,12 @@ def test_no_gpu_call_needed_for_plans():
         h = ctypes.c_void_p()
         rc = ms.lib().msosm_engine_create(p.h, None, ctypes.byref(h))
         assert rc != 0 and h.value is None
+
+
+def test_engine_rejects_a_ring_beyond_32bit_offsets():
+    # the shared-ring forms address the spectra ring with 32-bit byte offsets:
+    # 151 blocks of history + 40000 per call x 16384 points x 8 B > 4 GiB
+    # (checked before any device is touched)
+    p = ms.Plan(64e6, [100.0], 1368e6, fftpoint=16384, count=40000, form=ms.FORM_FUSED_F32)
+    with pytest.raises(ms.MsosmError, match="below 4 GiB"):
+        ms.Engine(p)
