@@ -5,6 +5,8 @@ import os
 import re
 import subprocess
 
+import pytest
+
 import msosm_gpu_amd as ms
 
 HEADER = os.path.join(ms.INCLUDE_DIR, "msosm_c.h")
