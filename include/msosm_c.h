@@ -144,7 +144,8 @@ int msosm_engine_stream(msosm_engine *eng, void **hip_stream);
  * host-to-device copy of host input is timed on the engine's copy stream).
  * Stage ids: MSOSM_STAGE_*.  times_ms[i] = summed kernel time of stage i,
  * calls[i] = launches timed.  COPY is the device-side overlap-save history
- * copy, H2D the copy of host input. */
+ * copy of the plain form (the other forms keep that history inside the
+ * forward transform), H2D the copy of host input. */
 #define MSOSM_STAGE_UNPACK 0
 #define MSOSM_STAGE_FFT_FWD 1
 #define MSOSM_STAGE_MULTIPLY 2

@@ -408,7 +408,7 @@ int create_engine(const msosm::Plan &plan, void *stream, msosm_engine **out)
             }
         }
         ALLOC(e->delay16, (size_t)ndm * N);
-        ALLOC(e->tail, M);
+        ALLOC(e->tail, 2 * (size_t)M); // two halves, alternating by call
         if (f16) {
             ALLOC(e->H16, (size_t)ndm * N);
             __half2 *r = nullptr;
@@ -421,7 +421,7 @@ int create_engine(const msosm::Plan &plan, void *stream, msosm_engine **out)
             e->xring = r;
             HIP_TRY(hipMemsetAsync(e->xring, 0, sizeof(float2) * (size_t)e->rx * N, e->stream));
         }
-        HIP_TRY(hipMemsetAsync(e->tail, 0, sizeof(uint32_t) * M, e->stream));
+        HIP_TRY(hipMemsetAsync(e->tail, 0, sizeof(uint32_t) * 2 * M, e->stream));
         int32_t *tmp = nullptr;
         ALLOC(tmp, (size_t)ndm * N);
         herr = hipMemcpyAsync(tmp, p.delay.data(), sizeof(int32_t) * ndm * N, hipMemcpyHostToDevice, e->stream);
@@ -518,6 +518,12 @@ int filter_plain(msosm_engine *e, const uint32_t *cur)
     return MSOSM_OK;
 }
 
+// Shared-ring forms keep the previous call's last M input samples in one
+// half of e->tail and write this call's into the other (the forward kernel
+// reads one while its last window writes the other), alternating by call.
+uint32_t *tail_in(msosm_engine *e) { return e->tail + (size_t)((e->g0 / e->plan.count) & 1) * e->plan.M; }
+uint32_t *tail_out(msosm_engine *e) { return e->tail + (size_t)(((e->g0 / e->plan.count) & 1) ^ 1) * e->plan.M; }
+
 // Hankel forms: shared forward spectra as in the fused forms, the
 // delay-line accumulate as per-bin GEMMs on MFMA into dense products, then
 // the inverse transform with the discard fused.
@@ -530,11 +536,8 @@ int filter_hankel(msosm_engine *e, const uint32_t *cur)
     hipStream_t s = e->stream;
     {
         StageScope st(e, MSOSM_STAGE_FFT_FWD);
-        HIP_TRY(msosm::launch_fwd_u16(cur, e->tail, e->xring, f16, e->log2n, count, e->g0, e->rx, false, s));
-    }
-    {
-        StageScope st(e, MSOSM_STAGE_COPY);
-        HIP_TRY(msosm::launch_copy_u32(e->tail, cur + (size_t)(count - 1) * M, M, s));
+        HIP_TRY(msosm::launch_fwd_u16(cur, tail_in(e), tail_out(e), e->xring, f16, e->log2n, count, e->g0, e->rx,
+                                      false, s));
     }
     {
         StageScope st(e, MSOSM_STAGE_MULTIPLY);
@@ -562,11 +565,8 @@ int filter_fused(msosm_engine *e, const uint32_t *cur)
     hipStream_t s = e->stream;
     {
         StageScope st(e, MSOSM_STAGE_FFT_FWD);
-        HIP_TRY(msosm::launch_fwd_u16(cur, e->tail, e->xring, f16, e->log2n, count, e->g0, e->rx, mfma, s));
-    }
-    {
-        StageScope st(e, MSOSM_STAGE_COPY);
-        HIP_TRY(msosm::launch_copy_u32(e->tail, cur + (size_t)(count - 1) * M, M, s));
+        HIP_TRY(msosm::launch_fwd_u16(cur, tail_in(e), tail_out(e), e->xring, f16, e->log2n, count, e->g0, e->rx,
+                                      mfma, s));
     }
     {
         StageScope st(e, MSOSM_STAGE_FFT_INV);
@@ -594,7 +594,7 @@ int msosm::engine_reset_async(msosm_engine *e)
     } else {
         size_t words = (form_f16_ring(e->form) ? 1 : 2) * (size_t)e->rx * p.N;
         HIP_TRY(msosm::launch_fill_u32(e->xring, words, 0, e->stream));
-        HIP_TRY(msosm::launch_fill_u32(e->tail, p.M, 0, e->stream));
+        HIP_TRY(msosm::launch_fill_u32(e->tail, 2 * (size_t)p.M, 0, e->stream));
         e->g0 = 0;
     }
     return MSOSM_OK;

@@ -19,9 +19,10 @@ size_t fft_c2c_scratch_elems(int log2n, long long batch);
 // Fused forward FFT of count overlapped windows read straight from packed
 // uint16 input (cur) and the previous call's last M samples (tail), written
 // to slots (g0 + w) mod rx of a spectra ring (FP32, or FP16 scaled by 1/N);
+// this call's last M samples go to tail_out (a different buffer from tail);
 // mfma_order (FP16, 2^14 points only): bin n at position mfft::ring_pos(n).
-hipError_t launch_fwd_u16(const uint32_t *cur, const uint32_t *tail, void *ring, bool f16, int log2n, int count,
-                          long long g0, int rx, bool mfma_order, hipStream_t s);
+hipError_t launch_fwd_u16(const uint32_t *cur, const uint32_t *tail, uint32_t *tail_out, void *ring, bool f16,
+                          int log2n, int count, long long g0, int rx, bool mfma_order, hipStream_t s);
 // Fused inverse: per (DM, block) gather X_{g-d(n)}[n] * H[n] from the ring,
 // inverse FFT, keep the last M samples, scale; out layout [dm][count][M].
 // dpk: for sizes with one transform per workgroup (fused_uses_packed_delay),

@@ -379,8 +379,8 @@ __global__ __launch_bounds__(cols_threads(LOG2N)) void k_fft_passB(
 // |x| <= 32768*sqrt(2) bounds |X|/N by 46341 < 65504).
 template <int LOG2N, bool F16, bool MFMA_ORDER = false>
 __global__ __launch_bounds__(fft_threads(LOG2N)) void k_fwd_u16(
-    const uint32_t *__restrict__ cur, const uint32_t *__restrict__ tail, void *__restrict__ ring, int count,
-    long long g0, int rx)
+    const uint32_t *__restrict__ cur, const uint32_t *__restrict__ tail, uint32_t *__restrict__ tail_out,
+    void *__restrict__ ring, int count, long long g0, int rx)
 {
     constexpr int E = fft_e(LOG2N);
     using S = FftShape<LOG2N, E>;
@@ -393,15 +393,30 @@ __global__ __launch_bounds__(fft_threads(LOG2N)) void k_fwd_u16(
         fill_twiddles<LOG2N, E>(twlo, twhi, threadIdx.x, blockDim.x);
     const int w = blockIdx.x * TPB + tl;
     const bool valid = w < count;
+    // window w covers input samples [(w - 1) M, (w + 1) M) of this call; the
+    // first M of window 0 are the previous call's last M (tail)
+    uint32_t u[E];
+    if (valid) {
+#pragma unroll
+        for (int m = 0; m < E; m++) {
+            const long long rel = (long long)(w - 1) * M + t + m * S::NT;
+            u[m] = *(rel < 0 ? tail + (M + rel) : cur + rel);
+        }
+        // the last window's second half is the next call's history
+        if (w == count - 1) {
+#pragma unroll
+            for (int m = E / 2; m < E; m++)
+                tail_out[t + m * S::NT - M] = u[m];
+        }
+    } else {
+#pragma unroll
+        for (int m = 0; m < E; m++)
+            u[m] = 0;
+    }
     float2 v[E];
 #pragma unroll
-    for (int m = 0; m < E; m++) {
-        long long rel = (long long)(w - 1) * M + t + m * S::NT;
-        uint32_t u = 0;
-        if (valid)
-            u = rel < 0 ? tail[M + rel] : cur[rel];
-        v[m] = unpack_u16pair(u);
-    }
+    for (int m = 0; m < E; m++)
+        v[m] = unpack_u16pair(u[m]);
     fft_block<LOG2N, E, false>(v, smem + tl * S::NPAD, twlo, twhi, t);
     if (valid) {
         const long long slot = (g0 + w) % rx;
@@ -742,8 +757,8 @@ hipError_t launch_fft_c2c(const float2 *in, float2 *out, int log2n, long long ba
 bool fused_supported(int log2n) { return log2n >= 2 && log2n <= kMaxLog2SinglePass; }
 
 template <int LOG2N>
-static hipError_t fwd_u16(const uint32_t *cur, const uint32_t *tail, void *ring, bool f16, int count, long long g0,
-                          int rx, bool mfma_order, hipStream_t s)
+static hipError_t fwd_u16(const uint32_t *cur, const uint32_t *tail, uint32_t *tail_out, void *ring, bool f16,
+                          int count, long long g0, int rx, bool mfma_order, hipStream_t s)
 {
     constexpr int E = fft_e(LOG2N);
     constexpr int threads = FftShape<LOG2N, E>::NT * fft_tpb(LOG2N);
@@ -752,8 +767,8 @@ static hipError_t fwd_u16(const uint32_t *cur, const uint32_t *tail, void *ring,
         if (mfma_order) {
             if (!f16)
                 return hipErrorInvalidValue;
-            hipLaunchKernelGGL((k_fwd_u16<LOG2N, true, true>), dim3(grid), dim3(threads), 0, s, cur, tail, ring,
-                               count, g0, rx);
+            hipLaunchKernelGGL((k_fwd_u16<LOG2N, true, true>), dim3(grid), dim3(threads), 0, s, cur, tail, tail_out,
+                               ring, count, g0, rx);
             MSOSM_LAUNCH_CHECK();
             return hipSuccess;
         }
@@ -761,11 +776,11 @@ static hipError_t fwd_u16(const uint32_t *cur, const uint32_t *tail, void *ring,
         return hipErrorInvalidValue;
     }
     if (f16)
-        hipLaunchKernelGGL((k_fwd_u16<LOG2N, true>), dim3(grid), dim3(threads), 0, s, cur, tail, ring, count, g0,
-                           rx);
+        hipLaunchKernelGGL((k_fwd_u16<LOG2N, true>), dim3(grid), dim3(threads), 0, s, cur, tail, tail_out, ring, count,
+                           g0, rx);
     else
-        hipLaunchKernelGGL((k_fwd_u16<LOG2N, false>), dim3(grid), dim3(threads), 0, s, cur, tail, ring, count, g0,
-                           rx);
+        hipLaunchKernelGGL((k_fwd_u16<LOG2N, false>), dim3(grid), dim3(threads), 0, s, cur, tail, tail_out, ring,
+                           count, g0, rx);
     MSOSM_LAUNCH_CHECK();
     return hipSuccess;
 }
@@ -864,10 +879,10 @@ static hipError_t inv_fused(const void *ring, const void *H, const uint16_t *del
     default: return hipErrorInvalidValue;                                                                        \
     }
 
-hipError_t launch_fwd_u16(const uint32_t *cur, const uint32_t *tail, void *ring, bool f16, int log2n, int count,
-                          long long g0, int rx, bool mfma_order, hipStream_t s)
+hipError_t launch_fwd_u16(const uint32_t *cur, const uint32_t *tail, uint32_t *tail_out, void *ring, bool f16,
+                          int log2n, int count, long long g0, int rx, bool mfma_order, hipStream_t s)
 {
-#define CALL_F(L) fwd_u16<L>(cur, tail, ring, f16, count, g0, rx, mfma_order, s)
+#define CALL_F(L) fwd_u16<L>(cur, tail, tail_out, ring, f16, count, g0, rx, mfma_order, s)
     MSOSM_DISPATCH_2_14(log2n, CALL_F)
 #undef CALL_F
 }
