@@ -174,7 +174,7 @@ struct StageScope {
                 harvest(e);
             a = pool_get(e);
             if (a)
-                hipEventRecord(a, s);
+                (void)hipEventRecord(a, s);
         }
     }
     ~StageScope()
@@ -182,7 +182,7 @@ struct StageScope {
         if (e->timing && a) {
             hipEvent_t b = pool_get(e);
             if (b) {
-                hipEventRecord(b, s);
+                (void)hipEventRecord(b, s);
                 e->pending.push_back({stage, a, b});
             } else {
                 e->pool.push_back(a);
@@ -196,27 +196,27 @@ void free_engine(msosm_engine *e)
     if (!e)
         return;
     if (e->stream)
-        hipStreamSynchronize(e->stream);
+        (void)hipStreamSynchronize(e->stream);
     harvest(e);
     for (auto ev : e->pool)
-        hipEventDestroy(ev);
+        (void)hipEventDestroy(ev);
     if (e->copy_stream) {
-        hipStreamSynchronize(e->copy_stream);
-        hipStreamDestroy(e->copy_stream);
+        (void)hipStreamSynchronize(e->copy_stream);
+        (void)hipStreamDestroy(e->copy_stream);
     }
     if (e->copy_done)
-        hipEventDestroy(e->copy_done);
+        (void)hipEventDestroy(e->copy_done);
     for (hipEvent_t ev : e->stage_free)
         if (ev)
-            hipEventDestroy(ev);
+            (void)hipEventDestroy(ev);
     void *bufs[] = {e->in_stage, e->H32,  e->H16,     e->delay32,  e->delay16, e->delay_pk, e->Hp,
                     e->dpk_mfma, e->mfma_tab, e->out, e->out_u16, e->inbuf,
                     e->fftblk, e->ring, e->ifftblk, e->fft_scratch, e->tail, e->xring, e->ybuf};
     for (void *b : bufs)
         if (b)
-            hipFree(b);
+            (void)hipFree(b);
     if (e->own_stream && e->stream)
-        hipStreamDestroy(e->stream);
+        (void)hipStreamDestroy(e->stream);
     delete e;
 }
 
@@ -242,17 +242,17 @@ int build_filter_tables(msosm_engine *e)
     float2 *c = nullptr, *h = nullptr, *pad = nullptr, *scratch = nullptr;
     int rc = MSOSM_OK;
     auto cleanup = [&]() {
-        hipStreamSynchronize(e->stream);
+        (void)hipStreamSynchronize(e->stream);
         if (prm_d)
-            hipFree(prm_d);
+            (void)hipFree(prm_d);
         if (c)
-            hipFree(c);
+            (void)hipFree(c);
         if (h)
-            hipFree(h);
+            (void)hipFree(h);
         if (pad)
-            hipFree(pad);
+            (void)hipFree(pad);
         if (scratch)
-            hipFree(scratch);
+            (void)hipFree(scratch);
     };
     auto fail = [&](hipError_t err, const char *what) {
         cleanup();

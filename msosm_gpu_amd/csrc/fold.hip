@@ -364,19 +364,19 @@ void fold_free(msosm_fold *f)
     if (!f)
         return;
     if (f->stream)
-        hipStreamSynchronize(f->stream);
+        (void)hipStreamSynchronize(f->stream);
     void *bufs[] = {f->intensity, f->raw, f->tmp, f->folded, f->starts_d};
     for (void *b : bufs)
         if (b)
-            hipFree(b);
+            (void)hipFree(b);
     for (int i = 0; i < 2; i++) {
         if (f->starts_h[i])
-            hipHostFree(f->starts_h[i]);
+            (void)hipHostFree(f->starts_h[i]);
         if (f->starts_ev[i])
-            hipEventDestroy(f->starts_ev[i]);
+            (void)hipEventDestroy(f->starts_ev[i]);
     }
     if (f->own_stream && f->stream)
-        hipStreamDestroy(f->stream);
+        (void)hipStreamDestroy(f->stream);
     delete f;
 }
 
