@@ -9,6 +9,7 @@ trial, and graph replay against eager execution.
     (K = 152), reference-structured FP32 form
   - many trials: 2^26 samples, 64 MHz, 256 trials on [0, 100], FFT 2^14,
     FP16 (vector ALU and matrix-core inverse) and FP32 fused forms
+  - wide delay span: DM up to 500 at 64 MHz, FFT 2^14 (K = 757), fused forms
   - 24 independent streams of different bandwidth (8-64 MHz), length
     (2^20-2^25 samples), trial count (1-64, DM 0-300) and autotuned FFT size,
     each phase-folded into 512 bins, in one hipGraph
@@ -106,6 +107,47 @@ def test_many_trials_full_stream(device):
         got = b[i, 62:64].cpu().numpy().reshape(-1)
         err = rel_l2(got, ref)
         print(f"  DM {dms[i]:.2f}: FP32 fused vs FP64, blocks {b_last + 62}..: {err:.2e}", flush=True)
+        assert err < 1e-5
+
+
+@pytest.mark.timeout(300)
+def test_wide_delay_span_dm500(device):
+    """DM up to 500 at 64 MHz with FFT 2^14: delays of up to K - 1 = 756
+    blocks, 2^24 samples, 64 trials; FP16 (vector ALU
+    and matrix-core inverse) against FP32 fused for every trial, FP32 against
+    the FP64 oracle on the last blocks."""
+    import torch
+    bw, f0 = 64e6, 1368e6
+    dms = np.linspace(0.0, 500.0, 64).astype(np.float32)
+    L = 1 << 24
+    d_in = device_input(L, 0x5EED0005, device)
+    outs = {}
+    for form in (ms.FORM_FUSED_F32, ms.FORM_FUSED_F16, ms.FORM_FUSED_F16_MFMA):
+        p = ms.Plan(bw, dms, f0, fftpoint=16384, count=64, form=form)
+        assert (p.M, p.K_max) == (8192, 757)
+        n = p.count * p.M
+        eng = ms.Engine(p, stream=torch.cuda.current_stream())
+        for c in range(L // n):
+            eng.filter_u16(d_in[c * n:(c + 1) * n])
+        y = torch.empty((len(dms), p.count, p.M), dtype=torch.complex64, device=device)
+        eng.output_c64(y)
+        torch.cuda.synchronize()
+        eng.close()
+        outs[form] = y
+        print(f"form {form}: {L // n} calls, ring {p.R} slots", flush=True)
+    b = outs[ms.FORM_FUSED_F32]
+    for form in (ms.FORM_FUSED_F16, ms.FORM_FUSED_F16_MFMA):
+        a = outs[form]
+        per_dm = (torch.linalg.vector_norm((a - b).reshape(len(dms), -1), dim=1) /
+                  torch.linalg.vector_norm(b.reshape(len(dms), -1), dim=1)).cpu().numpy()
+        print(f"form {form} vs FP32 fused, 64 trials: max rel L2 {per_dm.max():.2e}", flush=True)
+        assert per_dm.max() < 1e-3
+    M, K = 8192, 757
+    b_last = L // M - 64
+    for i in (0, 37, 63):
+        ref = oracle_blocks(d_in, M, K, b_last + 62, 2, bw, float(dms[i]), f0)
+        err = rel_l2(b[i, 62:64].cpu().numpy().reshape(-1), ref)
+        print(f"  DM {dms[i]:.1f}: FP32 fused vs FP64, blocks {b_last + 62}..: {err:.2e}", flush=True)
         assert err < 1e-5
 
 
