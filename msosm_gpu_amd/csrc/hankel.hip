@@ -343,6 +343,73 @@ __global__ __launch_bounds__(fft_threads(LOG2N)) void k_inv_dense(const void *__
     }
 }
 
+// The same for sizes with one transform per workgroup: NB consecutive units
+// (blocks of one trial) per workgroup, so the twiddle tables are filled once
+// per NB transforms, and 2^14 points take the middle pass's twiddles from the
+// row table.
+constexpr int kDenseBlocksPerWg = 8;
+template <int LOG2N, bool F16Y, int NB>
+__global__ __launch_bounds__((1 << LOG2N) / fft_e(LOG2N)) void k_inv_dense_nb(const void *__restrict__ Y_,
+                                                                            float2 *__restrict__ out,
+                                                                            long long units, float scale)
+{
+    constexpr int E = fft_e(LOG2N);
+    using S = FftShape<LOG2N, E>;
+    static_assert(S::NT >= 256, "one transform per workgroup");
+    constexpr long long N = S::N, M = S::N / 2;
+    constexpr int NT = S::NT;
+    constexpr bool ROWS = (LOG2N == 14 && E == 32);
+    __shared__ float2 smem[S::NPAD + S::TWLO + S::TWHI + (ROWS ? kTwRowElems : 0)];
+    float2 *twlo = smem + S::NPAD, *twhi = twlo + S::TWLO, *twrows = twhi + S::TWHI;
+    fill_twiddles<LOG2N, E>(twlo, twhi, threadIdx.x, blockDim.x);
+    if constexpr (ROWS)
+        fill_twrows(twrows, threadIdx.x, blockDim.x);
+    const int t = threadIdx.x;
+    const long long u0 = (long long)blockIdx.x * NB;
+    const int nb = (int)(units - u0 < NB ? units - u0 : NB);
+    using Raw = typename std::conditional<F16Y, __half2, float2>::type;
+    // 2^14 points: block i + 1 is loaded while block i is transformed (the
+    // products were just written, so these loads come from HBM); the
+    // workgroup's LDS already limits the CU to one, so the extra registers
+    // cost no occupancy (at smaller sizes they would)
+    constexpr bool PF = LOG2N == 14;
+    Raw raw[E];
+    if constexpr (PF) {
+        const Raw *src = reinterpret_cast<const Raw *>(Y_) + u0 * N + t;
+#pragma unroll
+        for (int m = 0; m < E; m++)
+            raw[m] = src[m * NT];
+    }
+    for (int i = 0; i < nb; i++) {
+        const long long u = u0 + i;
+        if constexpr (!PF) {
+            const Raw *src = reinterpret_cast<const Raw *>(Y_) + u * N + t;
+#pragma unroll
+            for (int m = 0; m < E; m++)
+                raw[m] = src[m * NT];
+        }
+        float2 v[E];
+#pragma unroll
+        for (int m = 0; m < E; m++) {
+            if constexpr (F16Y)
+                v[m] = __half22float2(raw[m]);
+            else
+                v[m] = raw[m];
+        }
+        if (PF && i + 1 < nb) {
+            const Raw *src = reinterpret_cast<const Raw *>(Y_) + (u + 1) * N + t;
+#pragma unroll
+            for (int m = 0; m < E; m++)
+                raw[m] = src[m * NT];
+        }
+        fft_block<LOG2N, E, true, ROWS>(v, smem, twlo, twhi, t, twrows);
+        float2 *dst = out + u * M;
+#pragma unroll
+        for (int m = E / 2; m < E; m++)
+            dst[t + m * NT - M] = cscale(v[m], scale);
+    }
+}
+
 template <int MODE>
 size_t hankel_lds_bytes(int U)
 {
@@ -383,6 +450,17 @@ hipError_t inv_dense(const void *Y, float2 *out, bool f16y, long long units, flo
 {
     constexpr int E = fft_e(LOG2N);
     constexpr int threads = FftShape<LOG2N, E>::NT * fft_tpb(LOG2N);
+    if constexpr (fft_tpb(LOG2N) == 1) {
+        constexpr int NB = kDenseBlocksPerWg;
+        const long long grid = (units + NB - 1) / NB;
+        if (f16y)
+            hipLaunchKernelGGL((k_inv_dense_nb<LOG2N, true, NB>), dim3((unsigned)grid), dim3(threads), 0, s, Y, out,
+                               units, scale);
+        else
+            hipLaunchKernelGGL((k_inv_dense_nb<LOG2N, false, NB>), dim3((unsigned)grid), dim3(threads), 0, s, Y, out,
+                               units, scale);
+        return hipGetLastError();
+    }
     const long long grid = (units + fft_tpb(LOG2N) - 1) / fft_tpb(LOG2N);
     if (f16y)
         hipLaunchKernelGGL((k_inv_dense<LOG2N, true>), dim3((unsigned)grid), dim3(threads), 0, s, Y, out, units,
