@@ -108,7 +108,7 @@ struct msosm_engine {
     float2 *fft_scratch = nullptr;  // two-pass FFT scratch
     int current_index = 0;
     // FUSED
-    uint32_t *tail = nullptr;       // previous call's last M input samples
+    uint32_t *tail = nullptr;       // 2 x M: previous call's last M input samples / this call's
     void *xring = nullptr;          // rx*2M spectra (float2 or half2)
     int rx = 0;
     // HANKEL (delay-line accumulate as per-bin GEMMs on MFMA)
