@@ -343,15 +343,18 @@ __global__ __launch_bounds__(fft_threads(LOG2N)) void k_inv_dense(const void *__
     }
 }
 
-// The same for sizes with one transform per workgroup: NB consecutive units
-// (blocks of one trial) per workgroup, so the twiddle tables are filled once
-// per NB transforms, and 2^14 points take the middle pass's twiddles from the
-// row table.
-constexpr int kDenseBlocksPerWg = 8;
-template <int LOG2N, bool F16Y, int NB>
+// The same for sizes with one transform per workgroup: nbpw consecutive
+// units (blocks of one trial) per workgroup, so the twiddle tables are filled
+// once per nbpw transforms, and 2^14 points take the middle pass's twiddles
+// from the row table.  nbpw is 16 while that leaves at least 512 workgroups,
+// else 8 (256 trials x 64 blocks of 2^14: FP16 rows 0.502 ms per launch
+// against 0.512 at 8).
+constexpr int kDenseBlocksPerWg = 8, kDenseBlocksPerWgWide = 16;
+constexpr long long kDenseWideMinGrid = 512;
+template <int LOG2N, bool F16Y>
 __global__ __launch_bounds__((1 << LOG2N) / fft_e(LOG2N)) void k_inv_dense_nb(const void *__restrict__ Y_,
                                                                             float2 *__restrict__ out,
-                                                                            long long units, float scale)
+                                                                            long long units, float scale, int nbpw)
 {
     constexpr int E = fft_e(LOG2N);
     using S = FftShape<LOG2N, E>;
@@ -365,8 +368,8 @@ __global__ __launch_bounds__((1 << LOG2N) / fft_e(LOG2N)) void k_inv_dense_nb(co
     if constexpr (ROWS)
         fill_twrows(twrows, threadIdx.x, blockDim.x);
     const int t = threadIdx.x;
-    const long long u0 = (long long)blockIdx.x * NB;
-    const int nb = (int)(units - u0 < NB ? units - u0 : NB);
+    const long long u0 = (long long)blockIdx.x * nbpw;
+    const int nb = (int)(units - u0 < nbpw ? units - u0 : nbpw);
     using Raw = typename std::conditional<F16Y, __half2, float2>::type;
     // 2^14 points: block i + 1 is loaded while block i is transformed (the
     // products were just written, so these loads come from HBM); the
@@ -451,14 +454,16 @@ hipError_t inv_dense(const void *Y, float2 *out, bool f16y, long long units, flo
     constexpr int E = fft_e(LOG2N);
     constexpr int threads = FftShape<LOG2N, E>::NT * fft_tpb(LOG2N);
     if constexpr (fft_tpb(LOG2N) == 1) {
-        constexpr int NB = kDenseBlocksPerWg;
-        const long long grid = (units + NB - 1) / NB;
+        const int nbpw = (units + kDenseBlocksPerWgWide - 1) / kDenseBlocksPerWgWide >= kDenseWideMinGrid
+                             ? kDenseBlocksPerWgWide
+                             : kDenseBlocksPerWg;
+        const long long grid = (units + nbpw - 1) / nbpw;
         if (f16y)
-            hipLaunchKernelGGL((k_inv_dense_nb<LOG2N, true, NB>), dim3((unsigned)grid), dim3(threads), 0, s, Y, out,
-                               units, scale);
+            hipLaunchKernelGGL((k_inv_dense_nb<LOG2N, true>), dim3((unsigned)grid), dim3(threads), 0, s, Y, out,
+                               units, scale, nbpw);
         else
-            hipLaunchKernelGGL((k_inv_dense_nb<LOG2N, false, NB>), dim3((unsigned)grid), dim3(threads), 0, s, Y, out,
-                               units, scale);
+            hipLaunchKernelGGL((k_inv_dense_nb<LOG2N, false>), dim3((unsigned)grid), dim3(threads), 0, s, Y, out,
+                               units, scale, nbpw);
         return hipGetLastError();
     }
     const long long grid = (units + fft_tpb(LOG2N) - 1) / fft_tpb(LOG2N);

@@ -526,19 +526,19 @@ __global__ __launch_bounds__(fft_threads(LOG2N)) void k_inv_fused(
 }
 
 // Fused inverse for one transform per workgroup (2^12..2^14 points): the
-// workgroup runs NB consecutive blocks of one DM trial and issues the gather
-// of block i+1 before the transform of block i, so the ring loads overlap the
-// LDS passes.  Only the prefetched ring values stay in registers across the
-// transform; H and the delays are re-read per block (L2 hits).  The delays
+// workgroup runs nbpw consecutive blocks of one DM trial (H and the delays
+// are re-read per block, L2 hits); PREFETCH (a tuning variant) issues the
+// gather of block i+1 before the transform of block i and holds those ring
+// values in registers across it.  The delays
 // come in a thread-major packed layout dpk[dm][j][t] = d(t + 2j NT) |
 // d(t + (2j+1) NT) << 16 so each thread reads E/2 coalesced words.  Work
 // items are ordered so that each XCD (workgroups are dealt to the 8 XCDs
 // round robin) gets a contiguous range of DM trials and keeps their tables in
 // its L2.
-template <int LOG2N, int E, bool F16, int NB, bool PREFETCH>
+template <int LOG2N, int E, bool F16, bool PREFETCH>
 __global__ __launch_bounds__((1 << LOG2N) / E) void k_inv_fused_nb(
     const void *__restrict__ ring, const void *__restrict__ H, const uint32_t *__restrict__ dpk,
-    float2 *__restrict__ out, int count, int ndm, long long g0, int rx)
+    float2 *__restrict__ out, int count, int ndm, long long g0, int rx, int nbpw)
 {
     using S = FftShape<LOG2N, E>;
     static_assert(S::NT >= 256, "one transform per workgroup");
@@ -555,7 +555,7 @@ __global__ __launch_bounds__((1 << LOG2N) / E) void k_inv_fused_nb(
     if constexpr (ROWS)
         fill_twrows(twrows, threadIdx.x, blockDim.x);
     const int t = threadIdx.x;
-    const int chunks = (count + NB - 1) / NB;
+    const int chunks = (count + nbpw - 1) / nbpw;
     const int G = gridDim.x;
     int w = blockIdx.x;
     if ((G & 7) == 0)
@@ -563,8 +563,8 @@ __global__ __launch_bounds__((1 << LOG2N) / E) void k_inv_fused_nb(
     if (w >= ndm * chunks)
         return; // the whole workgroup leaves together (w is uniform)
     const int dm = w / chunks;
-    const int b0 = (w - dm * chunks) * NB;
-    const int nb = min(NB, count - b0);
+    const int b0 = (w - dm * chunks) * nbpw;
+    const int nb = min(nbpw, count - b0);
     const char *ringB = reinterpret_cast<const char *>(ring);
     const T *hrow = reinterpret_cast<const T *>(H) + (long long)dm * N;
     const uint32_t *drow = dpk + (long long)dm * N / 2;
@@ -785,10 +785,24 @@ static hipError_t fwd_u16(const uint32_t *cur, const uint32_t *tail, uint32_t *t
     return hipSuccess;
 }
 
+// Blocks per workgroup of k_inv_fused_nb: 16 while that still gives at
+// least two workgroups per CU (kInvWideMinGrid), else 8.  For 256 trials x 64
+// blocks of 2^14 points (1024 workgroups at 16) 16 takes 0.508 ms per launch
+// (FP32 ring 0.623) against 0.519 (0.653) at 8 and 0.519 (0.632) at 32: a
+// DM trial's chunks share its filter and delay rows in L2, neighbouring
+// trials their ring entries, and fewer, longer workgroups turn over less
+// often.
 constexpr int kInvBlocksPerWg = 8;
+constexpr int kInvBlocksPerWgWide = 16;
+constexpr long long kInvWideMinGrid = 512;
+static int inv_blocks_per_wg(int count, int ndm)
+{
+    const long long wide = (long long)((count + kInvBlocksPerWgWide - 1) / kInvBlocksPerWgWide) * ndm;
+    return wide >= kInvWideMinGrid ? kInvBlocksPerWgWide : kInvBlocksPerWg;
+}
 
 // Inverse-kernel variant for the sizes with one transform per workgroup:
-// 0 = one (DM, block) per workgroup, 1 = kInvBlocksPerWg blocks per
+// 0 = one (DM, block) per workgroup, 1 = inv_blocks_per_wg blocks per
 // workgroup (default), 2 = as 1 with the next block's gather prefetched into
 // registers, 3 = as 2 with 16 points per thread for 2^14 (1024 threads),
 // 4 = as 1 with 16 points per thread for 2^14.
@@ -811,10 +825,11 @@ static void launch_nb(const void *ring, const void *H, const uint32_t *dpk, floa
                       long long g0, int rx, hipStream_t s)
 {
     constexpr int threads = (1 << LOG2N) / E;
-    const long long chunks = (count + kInvBlocksPerWg - 1) / kInvBlocksPerWg;
+    const int nbpw = inv_blocks_per_wg(count, ndm);
+    const long long chunks = (count + nbpw - 1) / nbpw;
     const long long grid = chunks * ndm;
-    hipLaunchKernelGGL((k_inv_fused_nb<LOG2N, E, F16, kInvBlocksPerWg, PF>), dim3((unsigned)grid), dim3(threads), 0,
-                       s, ring, H, dpk, out, count, ndm, g0, rx);
+    hipLaunchKernelGGL((k_inv_fused_nb<LOG2N, E, F16, PF>), dim3((unsigned)grid), dim3(threads), 0, s, ring, H, dpk,
+                       out, count, ndm, g0, rx, nbpw);
 }
 
 template <int LOG2N>
