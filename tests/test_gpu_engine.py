@@ -135,6 +135,27 @@ def test_block_split_invariance(device):
         np.testing.assert_array_equal(outs[0], outs[2])
 
 
+@pytest.mark.parametrize("form", [ms.FORM_FUSED_F32, ms.FORM_FUSED_F16])
+def test_blocks_per_workgroup_choice(device, form):
+    """With 512 trials the fused inverse runs 16 blocks per workgroup (here
+    20 blocks per call: a full chunk and a partial one); 8 trials alone run
+    8 per workgroup.  Each trial's output must not depend on that choice (bit
+    for bit), and must match the FP64 oracle."""
+    bw, f0 = 64e6, 1368e6
+    dms = np.linspace(0.0, 50.0, 512).astype(np.float32)
+    u = make_input(2 * 20 * 2048, 41)
+    big = ms.Plan(bw, dms, f0, fftpoint=4096, count=20, form=form)
+    small = ms.Plan(bw, dms[:8], f0, fftpoint=4096, count=20, form=form)
+    assert big.M == 2048 and big.K_max > 100
+    a = run_engine(big, u, device=device)
+    b = run_engine(small, u, device=device)
+    np.testing.assert_array_equal(a[:8], b)
+    for i in (0, 255, 511):
+        err = rel_l2(a[i], oracle_out(bw, float(dms[i]), f0, 2048, u))
+        print(f"form {form} DM {dms[i]:.2f}: rel L2 {err:.2e}", flush=True)
+        assert err < TOL[form]
+
+
 def test_host_and_device_input_agree(device):
     bw, dm, f0 = 16e6, 10.0, 1392e6
     u = make_input(32 * 512, 4)
