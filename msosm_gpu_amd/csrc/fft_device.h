@@ -33,6 +33,14 @@ __device__ __forceinline__ float2 cmulconj(float2 a, float2 b) // a * conj(b)
     return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
 }
 __device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+// Store of output that nothing reads again before the kernel ends: a
+// streaming (non-temporal) store, so that the output stream does not evict
+// the tables and spectra that the kernel's gathers re-read from L2.
+__device__ __forceinline__ void store_stream(float2 *p, float2 v)
+{
+    __builtin_nontemporal_store(v.x, &p->x);
+    __builtin_nontemporal_store(v.y, &p->y);
+}
 
 // cos(2 pi k / 64) at compile time (Taylor series on a reduced argument;
 // double precision, rounded to float by the caller).

@@ -116,6 +116,15 @@ __device__ __forceinline__ void split(float x, typename HankelMode<MODE>::Op &hi
     lo = HankelMode<MODE>::SPLIT ? to_op<Op>(x - from_op<Op>(hi)) : to_op<Op>(0.f);
 }
 
+// Product rows are read back only by the next kernel, from HBM (they are far
+// larger than the caches): streaming stores keep this kernel's ring columns
+// and filter rows in L2.
+__device__ __forceinline__ void store_row(__half2 *p, __half2 v)
+{
+    __builtin_nontemporal_store(__builtin_bit_cast(uint32_t, v), reinterpret_cast<uint32_t *>(p));
+}
+__device__ __forceinline__ void store_row(float2 *p, float2 v) { store_stream(p, v); }
+
 // One workgroup: NBIN consecutive bins x kGTile consecutive output blocks,
 // looping over all DM tiles.  LDS (dynamic):
 //   z[P][NBIN][U]   ring column parts, u = kGTile-1 - (g - gb) + k
@@ -290,7 +299,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_hankel(const void *__restrict__
             const int i = r / kGTile, g = r % kGTile;
             const int dm = tile * kDmTile + i;
             if (dm < ndm && gb + g < count)
-                Y[((long long)dm * count + gb + g) * N + n0 + nl] = ys[r * YS_ROW + nl];
+                store_row(&Y[((long long)dm * count + gb + g) * N + n0 + nl], ys[r * YS_ROW + nl]);
         }
     }
 }
@@ -339,7 +348,7 @@ __global__ __launch_bounds__(fft_threads(LOG2N)) void k_inv_dense(const void *__
         float2 *dst = out + u * M;
 #pragma unroll
         for (int m = E / 2; m < E; m++)
-            dst[t + m * S::NT - M] = cscale(v[m], scale);
+            store_stream(&dst[t + m * S::NT - M], cscale(v[m], scale));
     }
 }
 
@@ -409,7 +418,7 @@ __global__ __launch_bounds__((1 << LOG2N) / fft_e(LOG2N)) void k_inv_dense_nb(co
         float2 *dst = out + u * M;
 #pragma unroll
         for (int m = E / 2; m < E; m++)
-            dst[t + m * NT - M] = cscale(v[m], scale);
+            store_stream(&dst[t + m * NT - M], cscale(v[m], scale));
     }
 }
 

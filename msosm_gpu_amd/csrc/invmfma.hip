@@ -276,8 +276,8 @@ __global__ __launch_bounds__(kThreads, 2) void k_inv_mfma(const __half2 *__restr
         for (int gi = 0; gi < 8; gi++) {
             const int jpp = 16 * (8 * wave + gi) + c;
             const f32x4 acc = mma(A16, bb[gi], f32x4{0.f, 0.f, 0.f, 0.f});
-            dst[jpp + 1024 * (2 * q)] = make_float2(acc[0], acc[1]);
-            dst[jpp + 1024 * (2 * q + 1)] = make_float2(acc[2], acc[3]);
+            store_stream(&dst[jpp + 1024 * (2 * q)], make_float2(acc[0], acc[1]));
+            store_stream(&dst[jpp + 1024 * (2 * q + 1)], make_float2(acc[2], acc[3]));
         }
         __syncthreads(); // every wave has read A2 before the next block's pass 0 writes A1
     }

@@ -521,7 +521,7 @@ __global__ __launch_bounds__(fft_threads(LOG2N)) void k_inv_fused(
         float2 *dst = out + ((long long)dm * count + b) * M;
 #pragma unroll
         for (int m = E / 2; m < E; m++)
-            dst[t + m * S::NT - M] = F16 ? v[m] : cscale(v[m], scale);
+            store_stream(&dst[t + m * S::NT - M], F16 ? v[m] : cscale(v[m], scale));
     }
 }
 
@@ -613,7 +613,7 @@ __global__ __launch_bounds__((1 << LOG2N) / E) void k_inv_fused_nb(
         float2 *dst = out + ((long long)dm * count + b0 + i) * M;
 #pragma unroll
         for (int m = E / 2; m < E; m++)
-            dst[t + m * NT - M] = F16 ? v[m] : cscale(v[m], scale);
+            store_stream(&dst[t + m * NT - M], F16 ? v[m] : cscale(v[m], scale));
     }
 }
 
