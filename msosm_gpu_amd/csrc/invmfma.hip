@@ -51,7 +51,8 @@ typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kThreads = 512; // 8 waves
 // blocks per workgroup: 16 while that leaves at least 512 workgroups, else 8
-// (256 trials x 64 blocks: 0.653 ms per launch against 0.658 at 8)
+// (256 trials x 64 blocks: 0.653 ms per launch against 0.658 at 8; with
+// streaming output stores 0.614 at 16 against 0.630 at 32)
 constexpr int kNB = 8, kNBWide = 16;
 constexpr long long kWideMinGrid = 512;
 constexpr int kTiles = 9;     // DFT_32: 4 M-tiles x 2 K-tiles; DFT_16: M-tile 1

@@ -785,20 +785,19 @@ static hipError_t fwd_u16(const uint32_t *cur, const uint32_t *tail, uint32_t *t
     return hipSuccess;
 }
 
-// Blocks per workgroup of k_inv_fused_nb: 16 while that still gives at
-// least two workgroups per CU (kInvWideMinGrid), else 8.  For 256 trials x 64
-// blocks of 2^14 points (1024 workgroups at 16) 16 takes 0.508 ms per launch
-// (FP32 ring 0.623) against 0.519 (0.653) at 8 and 0.519 (0.632) at 32: a
-// DM trial's chunks share its filter and delay rows in L2, neighbouring
-// trials their ring entries, and fewer, longer workgroups turn over less
-// often.
-constexpr int kInvBlocksPerWg = 8;
-constexpr int kInvBlocksPerWgWide = 16;
-constexpr long long kInvWideMinGrid = 512;
+// Blocks per workgroup of k_inv_fused_nb: the largest of 32, 16 and 8 that
+// still gives at least two workgroups per CU (kInvMinGrid).  A trial's chunks
+// share its filter and delay rows in L2, neighbouring trials their ring
+// entries, and fewer, longer workgroups turn over less often.  For 256
+// trials x 64 blocks of 2^14 points with streaming output stores: 0.463 ms
+// per launch at 32 (FP32 ring 0.578), 0.474 (0.591) at 16, 0.463 at 64.
+constexpr long long kInvMinGrid = 512;
 static int inv_blocks_per_wg(int count, int ndm)
 {
-    const long long wide = (long long)((count + kInvBlocksPerWgWide - 1) / kInvBlocksPerWgWide) * ndm;
-    return wide >= kInvWideMinGrid ? kInvBlocksPerWgWide : kInvBlocksPerWg;
+    for (int nbpw = 32; nbpw > 8; nbpw /= 2)
+        if ((long long)((count + nbpw - 1) / nbpw) * ndm >= kInvMinGrid)
+            return nbpw;
+    return 8;
 }
 
 // Inverse-kernel variant for the sizes with one transform per workgroup:
