@@ -156,14 +156,15 @@ def main():
     units = len(dms) * plan_info["count"]
     alg_bytes = units * plan_info["M"] * (8 + 8)  # gather 2 FP16 bins + write 1 complex64 per DM-sample
     achieved = alg_bytes / (st["ms_per_launch"] / 1e3) / 1e9
-    roofline = dict(bound="hbm", kernel="k_inv_fused<14,fp16>", achieved=round(achieved, 1), peak=HBM_PEAK_GBS,
+    kname = f"k_inv_fused_nb<{(2 * plan_info['M']).bit_length() - 1},fp16>"
+    roofline = dict(bound="hbm", kernel=kname, achieved=round(achieved, 1), peak=HBM_PEAK_GBS,
                     unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 4),
                     traffic=measured_traffic(args, len(dms)),
                     alg_bytes_per_launch=alg_bytes, ms_per_launch=st["ms_per_launch"])
     kernels = kernel_rooflines(results, plan_info, len(dms), fold, hankel_flops)
     copy_gbs = copy_bandwidth(torch, dev)
     log(f"device copy (1 GiB read + 1 GiB write): {copy_gbs:.0f} GB/s")
-    kernels["k_inv_fused<14,fp16>"] = {k: roofline[k] for k in ("achieved", "unit", "peak", "frac", "ms_per_launch")}
+    kernels[kname] = {k: roofline[k] for k in ("achieved", "unit", "peak", "frac", "ms_per_launch")}
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(bw, f0, dms, args.fftpoint // 2, u, args.cpu_log2_samples)
