@@ -123,7 +123,7 @@ __global__ void k_multiply_delay(const float2 *__restrict__ a, const float2 *__r
         size_t off = ((size_t)(j / count) * ndm + dm) * ((size_t)count * N) + (size_t)(j % count) * N + n;
         float2 x = arow[n], h = brow[n];
         float2 r = make_float2((x.x * h.x - x.y * h.y) * invN, (x.x * h.y + x.y * h.x) * invN);
-        ring[off] = r;
+        store_stream(&ring[off], r); // read back calls later, from HBM
     }
 }
 
@@ -135,7 +135,7 @@ __global__ void k_discard(const float2 *__restrict__ a, float2 *__restrict__ b, 
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
         size_t g = i / M, o = i % M;
         float2 v = a[g * 2 * M + M + o];
-        b[i] = make_float2(v.x * scale, v.y * scale);
+        store_stream(&b[i], make_float2(v.x * scale, v.y * scale));
     }
 }
 
