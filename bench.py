@@ -53,7 +53,7 @@ def parse():
                     help="short run for hardware-counter passes: a few calls of the headline form and a 1 GiB "
                          "device copy (the counter calibration), no timing")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-check", action="store_true", help="skip the FP64 spot check")
+    ap.add_argument("--no-check", action="store_true", help="skip the comparison with the plain FP32 output")
     ap.add_argument("--cpu-log2-samples", type=int, default=23)
     return ap.parse_args()
 
@@ -88,6 +88,7 @@ def main():
     stream = torch.cuda.current_stream()
     results = {}
     checks = {}
+    plain_ref = None
     form_ids = {"plain_f32": ms.FORM_PLAIN_F32, "mixed_f16": ms.FORM_FUSED_F16,
                 "mixed_f16_mfma": ms.FORM_FUSED_F16_MFMA, "hankel_f16": ms.FORM_HANKEL_F16,
                 "hankel_bf16x3": ms.FORM_HANKEL_BF16X3}
@@ -144,9 +145,13 @@ def main():
                 f"{fold['achieved']:.0f} GB/s")
         if name.startswith("hankel") and hankel_flops is None:
             hankel_flops = hankel_flop_count(plan)
-        if not args.no_check:
-            checks[name] = spot_check(ms, eng, plan, d_in, u, bw, f0, dms, n)
-            log(f"{name}: max rel L2 vs FP64 on spot check = {checks[name]['max_rel_l2']:.3e}")
+        if not args.no_check and (name == "plain_f32" or plain_ref is not None):
+            blocks = check_blocks(eng, plan, d_in, n, len(dms))
+            if name == "plain_f32":
+                plain_ref = blocks
+            else:
+                checks[name] = compare_blocks(blocks, plain_ref, dms)
+                log(f"{name}: max rel L2 vs plain FP32 on the check blocks = {checks[name]['max_rel_l2']:.3e}")
         eng.close()
         del eng
         torch.cuda.synchronize()
@@ -198,7 +203,7 @@ def main():
         "kernels": kernels,
         "hbm_copy_gbs": round(copy_gbs, 1),
         "host_input_pcie": pcie,
-        "rel_l2_vs_fp64": {k: v for k, v in checks.items()},
+        "rel_l2_vs_plain_f32": {k: v for k, v in checks.items()},
     }
     print(json.dumps(line), flush=True)
 
@@ -369,27 +374,33 @@ def kernel_rooflines(results, plan_info, ndm, fold, hankel_flops):
     return out
 
 
-def spot_check(ms, eng, plan, d_in, u, bw, f0, dms, n):
-    """Re-run from a reset engine up to call c and compare its first blocks
-    for a few DMs against the FP64 same-plan oracle."""
-    import oracle
+def check_dms(ndm):
+    """DM trial indices of the check blocks."""
+    return sorted({min(i, ndm - 1) for i in (0, 37, 146, ndm - 1)})
+
+
+def check_blocks(eng, plan, d_in, n, ndm):
+    """Re-run from a reset engine up to call 2 and keep that call's first 4
+    blocks for a few DM trials (the same blocks for every form)."""
     eng.reset()
     c_chk = 2
     for c in range(c_chk + 1):
         eng.filter_u16(d_in[c * n:(c + 1) * n])
     out = eng.output_c64()
-    M = plan.M
     nb = 4
-    b0 = c_chk * plan.count
-    x = oracle.unpack_u16(u[:2 * (b0 + nb) * M]).astype(np.complex128)
+    return dict(blocks=[c_chk * plan.count, c_chk * plan.count + nb],
+                rows={i: out[i, :nb].reshape(-1).astype(np.complex128) for i in check_dms(ndm)})
+
+
+def compare_blocks(blocks, ref, dms):
+    """Relative L2 of a form's check blocks against the plain FP32 form's --
+    the condition the metric is quoted under (the FP32 forms' own parity with
+    the FP64 oracle is established by the test suite)."""
     errs = {}
-    for i in (0, 37, 146, len(dms) - 1):
-        d, K, _ = oracle.delay_points(bw, float(dms[i]), f0, M)
-        H = oracle.dedisp_params(bw, float(dms[i]), f0, M)
-        ref = oracle.msosm_run(x, M, d, H, b0=b0, nb=nb)
-        got = out[i, :nb].reshape(-1)
-        errs[f"dm{float(dms[i]):.2f}"] = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
-    return dict(blocks=[b0, b0 + nb], per_dm=errs, max_rel_l2=max(errs.values()))
+    for i, r in ref["rows"].items():
+        got = blocks["rows"][i]
+        errs[f"dm{float(dms[i]):.2f}"] = float(np.linalg.norm(got - r) / np.linalg.norm(r))
+    return dict(blocks=ref["blocks"], per_dm=errs, max_rel_l2=max(errs.values()))
 
 
 def cpu_baseline(bw, f0, dms, M, u, log2_samples, reps=5, log2_single=20):
