@@ -105,3 +105,19 @@ def test_mfma_full_scale_inputs(device, kind):
     err = rel_l2(y, oracle_out(BW, dm, F0, 8192, u))
     print(f"mfma {kind}: rel L2 {err:.2e}", flush=True)
     assert err < MIXED_TOL
+
+
+def test_mfma_blocks_per_workgroup_choice(device):
+    # 512 trials run 16 blocks per workgroup (20 per call: a full chunk and a
+    # partial one), 8 trials alone run 8: each trial's output must not depend
+    # on that choice (bit for bit)
+    dms = np.linspace(0.0, 50.0, 512).astype(np.float32)
+    u = make_input(2 * 20 * 8192, 43)
+    big = ms.Plan(BW, dms, F0, fftpoint=16384, count=20, form=ms.FORM_FUSED_F16_MFMA)
+    small = ms.Plan(BW, dms[:8], F0, fftpoint=16384, count=20, form=ms.FORM_FUSED_F16_MFMA)
+    a = run_engine(big, u, device=device)
+    b = run_engine(small, u, device=device)
+    np.testing.assert_array_equal(a[:8], b)
+    err = rel_l2(a[511], oracle_out(BW, float(dms[511]), F0, 8192, u))
+    print(f"mfma 512 trials, DM 50: rel L2 {err:.2e}", flush=True)
+    assert err < MIXED_TOL
