@@ -61,15 +61,20 @@ struct Core {
         d.form = form;
         check(msosm_plan_create(&d, &plan), "plan");
         check(msosm_plan_get_info(plan, &info), "plan info");
-        if (d.form != MSOSM_FORM_PLAIN_F32 && info.fftpoint > 16384) {
-            // the fused kernels hold one transform in LDS; larger sizes use the
-            // reference-structured form
+        int rc = MSOSM_ERR_UNSUPPORTED;
+        // the fused kernels hold one transform in LDS (FFT sizes up to 16384),
+        // keep delays in 16 bits and address their spectra ring with 32-bit
+        // offsets; plans beyond that use the reference-structured form
+        if (d.form == MSOSM_FORM_PLAIN_F32 || info.fftpoint <= 16384)
+            rc = msosm_engine_create(plan, nullptr, &eng);
+        if (rc == MSOSM_ERR_UNSUPPORTED && d.form != MSOSM_FORM_PLAIN_F32) {
             msosm_plan_destroy(plan);
             d.form = MSOSM_FORM_PLAIN_F32;
             check(msosm_plan_create(&d, &plan), "plan");
             check(msosm_plan_get_info(plan, &info), "plan info");
+            rc = msosm_engine_create(plan, nullptr, &eng);
         }
-        check(msosm_engine_create(plan, nullptr, &eng), "engine");
+        check(rc, "engine");
     }
     void destroy()
     {
