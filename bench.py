@@ -55,6 +55,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-check", action="store_true", help="skip the comparison with the plain FP32 output")
     ap.add_argument("--cpu-log2-samples", type=int, default=23)
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the short runs of the other workloads (single trial at 16 and 128 MHz, 24 streams)")
     return ap.parse_args()
 
 
@@ -174,6 +176,9 @@ def main():
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(bw, f0, dms, args.fftpoint // 2, u, args.cpu_log2_samples)
         log(f"cpu baseline: {cpu}")
+    others = None if args.no_configs else other_configs(ms, torch, dev)
+    if others:
+        log(f"other workloads: {json.dumps(others)}")
     line = {
         "metric": METRIC,
         "value": round(mixed["value"], 2),
@@ -204,8 +209,110 @@ def main():
         "hbm_copy_gbs": round(copy_gbs, 1),
         "host_input_pcie": pcie,
         "rel_l2_vs_plain_f32": {k: v for k, v in checks.items()},
+        "other_configs": others,
     }
     print(json.dumps(line), flush=True)
+
+
+def random_u16_words(torch, dev, L, seed):
+    """L packed offset-binary uint16 pairs (codes 1..65535) on the device."""
+    g = torch.Generator(device=dev).manual_seed(seed)
+    lo = torch.randint(1, 65536, (L,), generator=g, device=dev, dtype=torch.int32)
+    hi = torch.randint(1, 65536, (L,), generator=g, device=dev, dtype=torch.int32)
+    return lo | (hi << 16)
+
+
+def time_engine(ms, torch, plan, d_in, L, steps=2, warmup=1):
+    """DM-trial*Msamples/s of one engine over the first whole calls of d_in."""
+    stream = torch.cuda.current_stream()
+    eng = ms.Engine(plan, stream=stream)
+    n = plan.count * plan.M
+    calls = L // n
+
+    def step():
+        for c in range(calls):
+            eng.filter_u16(d_in[c * n:(c + 1) * n])
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(steps):
+        step()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms_step = e0.elapsed_time(e1) / steps
+    eng.close()
+    return dict(value=round(plan.num_dms * calls * n / (ms_step / 1e3) / 1e6, 1), ms_per_step=round(ms_step, 3),
+                samples=calls * n, fftpoint=2 * plan.M, count=plan.count)
+
+
+def other_configs(ms, torch, dev):
+    """Short runs of the other workloads (inputs resident in HBM, uniform
+    random codes): a single trial at 16 MHz (DM 10, FFT 1024) and at 128 MHz
+    (DM 100, FFT 2^16, which only the reference-structured form takes), and
+    24 independent streams of 8-64 MHz, 2^20-2^25 samples, 1-64 trials,
+    autotuned FFT sizes, each phase-folded, replayed as one graph."""
+    out = {}
+    L = 1 << 22
+    d_in = random_u16_words(torch, dev, L, 0x5EED0001)
+    c1 = {}
+    for name, form in (("mixed_f16", ms.FORM_FUSED_F16), ("fused_f32", ms.FORM_FUSED_F32),
+                       ("plain_f32", ms.FORM_PLAIN_F32)):
+        c1[name] = time_engine(ms, torch, ms.Plan(16e6, 10.0, 1392e6, fftpoint=1024, count=1024, form=form), d_in,
+                               L, steps=3)
+    out["single trial, 16 MHz, DM 10, 2^22 samples, FFT 1024, 1024 blocks per call"] = c1
+    del d_in
+    L = 1 << 28
+    d_in = random_u16_words(torch, dev, L, 0x5EED0002)
+    c2 = {"plain_f32, FFT 2^16 (K 152)": time_engine(ms, torch, ms.Plan(128e6, 100.0, 1336e6, fftpoint=65536,
+                                                                          count=32, form=ms.FORM_PLAIN_F32),
+                                                     d_in, L, steps=2)}
+    for name, form in (("mixed_f16", ms.FORM_FUSED_F16), ("fused_f32", ms.FORM_FUSED_F32)):
+        c2[name + ", FFT 2^14 (K 608)"] = time_engine(ms, torch, ms.Plan(128e6, 100.0, 1336e6, fftpoint=16384,
+                                                                         count=64, form=form), d_in, L, steps=2)
+    out["single trial, 128 MHz, DM 100, 2^28 samples"] = c2
+    del d_in
+    torch.cuda.empty_cache()
+    rng = np.random.Generator(np.random.Philox(0x5EED0004))
+    engines, inputs, lens, periods, work = [], [], [], [], 0
+    for s in range(24):
+        bw = float(rng.choice([8e6, 16e6, 32e6, 64e6]))
+        f0 = 1400e6 - bw / 2
+        Ls = 1 << int(rng.integers(20, 26))
+        ndm = int(rng.integers(1, 65))
+        dms = np.sort(rng.uniform(0, 300, ndm)).astype(np.float32)
+        period = float(np.exp(rng.uniform(np.log(1e-3), np.log(1.0))))
+        fp, _, form = ms.autotune_fftpoint(bw, dms, f0, min_fftpoint=1024, max_fftpoint=65536,
+                                           target_samples=min(Ls, 1 << 19), calls=1, form=ms.FORM_FUSED_F16)
+        M = fp // 2
+        count = max(1, min(64, Ls // M))
+        plan = ms.Plan(bw, dms, f0, fftpoint=fp, count=count, form=form)
+        n = plan.count * plan.M
+        used = (Ls // n) * n
+        engines.append(ms.Engine(plan))
+        inputs.append(random_u16_words(torch, dev, used, 1000 + s))
+        lens.append(used)
+        periods.append(period * bw)
+        work += ndm * used
+    torch.cuda.synchronize()
+    batch = ms.Batch(engines, inputs, lens, periods, nbins=512)
+    batch.launch()
+    batch.synchronize()
+    reps = 3
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        batch.launch()
+    batch.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    out["24 streams, 8-64 MHz, one graph with phase folds"] = {
+        "value": round(work / dt / 1e6, 1), "ms_per_pass": round(dt * 1e3, 3), "dm_trial_samples": work,
+        "graph_nodes": batch.graph_nodes}
+    batch.close()
+    for e in engines:
+        e.close()
+    return out
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_multidm_2e14_256dm.md")

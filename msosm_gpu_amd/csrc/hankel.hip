@@ -33,6 +33,7 @@
 #include <hip/hip_runtime.h>
 #include <hip/hip_fp16.h>
 
+#include <algorithm>
 #include <climits>
 #include <cstdint>
 #include <type_traits>
@@ -355,12 +356,10 @@ __global__ __launch_bounds__(fft_threads(LOG2N)) void k_inv_dense(const void *__
 // The same for sizes with one transform per workgroup: nbpw consecutive
 // units (blocks of one trial) per workgroup, so the twiddle tables are filled
 // once per nbpw transforms, and 2^14 points take the middle pass's twiddles
-// from the row table.  nbpw is the largest of 32, 16 and 8 that leaves at
-// least 512 workgroups (256 trials x 64 blocks of 2^14, streaming stores:
+// from the row table.  nbpw is the largest power of two up to 32 that leaves
+// at least 512 workgroups (256 trials x 64 blocks of 2^14, streaming stores:
 // FP16 rows 0.481 ms per launch at 32, 0.487 at 16, 0.512 at 8 before the
 // streaming stores).
-constexpr int kDenseBlocksPerWg = 8;
-constexpr long long kDenseWideMinGrid = 512;
 template <int LOG2N, bool F16Y>
 __global__ __launch_bounds__((1 << LOG2N) / fft_e(LOG2N)) void k_inv_dense_nb(const void *__restrict__ Y_,
                                                                             float2 *__restrict__ out,
@@ -464,12 +463,7 @@ hipError_t inv_dense(const void *Y, float2 *out, bool f16y, long long units, flo
     constexpr int E = fft_e(LOG2N);
     constexpr int threads = FftShape<LOG2N, E>::NT * fft_tpb(LOG2N);
     if constexpr (fft_tpb(LOG2N) == 1) {
-        int nbpw = kDenseBlocksPerWg;
-        for (int c = 32; c > kDenseBlocksPerWg; c /= 2)
-            if ((units + c - 1) / c >= kDenseWideMinGrid) {
-                nbpw = c;
-                break;
-            }
+        const int nbpw = blocks_per_wg((int)std::min<long long>(units, INT_MAX), 1, 32);
         const long long grid = (units + nbpw - 1) / nbpw;
         if (f16y)
             hipLaunchKernelGGL((k_inv_dense_nb<LOG2N, true>), dim3((unsigned)grid), dim3(threads), 0, s, Y, out,

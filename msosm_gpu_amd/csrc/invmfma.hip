@@ -50,11 +50,10 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kThreads = 512; // 8 waves
-// blocks per workgroup: 16 while that leaves at least 512 workgroups, else 8
-// (256 trials x 64 blocks: 0.653 ms per launch against 0.658 at 8; with
-// streaming output stores 0.614 at 16 against 0.630 at 32)
-constexpr int kNB = 8, kNBWide = 16;
-constexpr long long kWideMinGrid = 512;
+// blocks per workgroup: the largest power of two up to 16 that leaves at
+// least 512 workgroups (256 trials x 64 blocks: 0.653 ms per launch against
+// 0.658 at 8; with streaming output stores 0.614 at 16 against 0.630 at 32)
+constexpr int kNBWide = 16;
 constexpr int kTiles = 9;     // DFT_32: 4 M-tiles x 2 K-tiles; DFT_16: M-tile 1
 constexpr int kTwLo = 128, kTwHi = 128; // W_N^i and W_N^(128 i), i < 128
 // FP16 headroom: the filter rows are stored divided by 64 (a power of two,
@@ -321,7 +320,7 @@ hipError_t launch_inv_mfma_filter(__half2 *dst, const __half2 *src, long long ro
 hipError_t launch_inv_mfma(const __half2 *ring, const __half2 *Hp, const uint32_t *dpk, const void *tables,
                            float2 *out, int count, int ndm, long long g0, int rx, hipStream_t s)
 {
-    const int nbpw = (long long)((count + kNBWide - 1) / kNBWide) * ndm >= kWideMinGrid ? kNBWide : kNB;
+    const int nbpw = blocks_per_wg(count, ndm, kNBWide);
     const long long chunks = (count + nbpw - 1) / nbpw;
     const long long grid = chunks * ndm;
     hipLaunchKernelGGL(k_inv_mfma, dim3((unsigned)grid), dim3(kThreads), 0, s, ring, Hp, dpk,

@@ -28,6 +28,10 @@ hipError_t launch_fwd_u16(const uint32_t *cur, const uint32_t *tail, uint32_t *t
 // dpk: for sizes with one transform per workgroup (fused_uses_packed_delay),
 // the delays in thread-major packed layout dpk[dm][j][t] = d(t + 2j*NT) |
 // d(t + (2j+1)*NT) << 16 with NT = fused_delay_threads(log2n); else unused.
+// Blocks per workgroup for the inverse kernels that run consecutive blocks
+// of one trial per workgroup: the largest power of two up to max_nbpw that
+// leaves at least 512 workgroups for rows x count blocks, else 1.
+int blocks_per_wg(int count, long long rows, int max_nbpw);
 hipError_t launch_inv_fused(const void *ring, const void *H, const uint16_t *delay, const uint32_t *dpk, float2 *out,
                             bool f16, int log2n, int count, int ndm, long long g0, int rx, hipStream_t s);
 int fused_delay_threads(int log2n);

@@ -785,20 +785,22 @@ static hipError_t fwd_u16(const uint32_t *cur, const uint32_t *tail, uint32_t *t
     return hipSuccess;
 }
 
-// Blocks per workgroup of k_inv_fused_nb: the largest of 32, 16 and 8 that
-// still gives at least two workgroups per CU (kInvMinGrid).  A trial's chunks
-// share its filter and delay rows in L2, neighbouring trials their ring
-// entries, and fewer, longer workgroups turn over less often.  For 256
-// trials x 64 blocks of 2^14 points with streaming output stores: 0.463 ms
-// per launch at 32 (FP32 ring 0.578), 0.474 (0.591) at 16, 0.463 at 64.
+// Blocks per workgroup of k_inv_fused_nb: the largest of 32, 16, ..., 1 that
+// still gives at least two workgroups per CU (kInvMinGrid), else 1.  A
+// trial's chunks share its filter and delay rows in L2, neighbouring trials
+// their ring entries, and fewer, longer workgroups turn over less often; few
+// trials need short chunks to fill the GPU at all.  For 256 trials x 64
+// blocks of 2^14 points with streaming output stores: 0.463 ms per launch at
+// 32 (FP32 ring 0.578), 0.474 (0.591) at 16, 0.463 at 64.
 constexpr long long kInvMinGrid = 512;
-static int inv_blocks_per_wg(int count, int ndm)
+int blocks_per_wg(int count, long long rows, int max_nbpw)
 {
-    for (int nbpw = 32; nbpw > 8; nbpw /= 2)
-        if ((long long)((count + nbpw - 1) / nbpw) * ndm >= kInvMinGrid)
+    for (int nbpw = max_nbpw; nbpw > 1; nbpw /= 2)
+        if ((long long)((count + nbpw - 1) / nbpw) * rows >= kInvMinGrid)
             return nbpw;
-    return 8;
+    return 1;
 }
+static int inv_blocks_per_wg(int count, int ndm) { return blocks_per_wg(count, ndm, 32); }
 
 // Inverse-kernel variant for the sizes with one transform per workgroup:
 // 0 = one (DM, block) per workgroup, 1 = inv_blocks_per_wg blocks per

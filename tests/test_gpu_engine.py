@@ -138,10 +138,9 @@ def test_block_split_invariance(device):
 @pytest.mark.parametrize("form", [ms.FORM_FUSED_F32, ms.FORM_FUSED_F16])
 def test_blocks_per_workgroup_choice(device, form):
     """With 512 trials the fused inverse runs 32 blocks per workgroup (here
-    20 blocks per call: one partial chunk); 8 trials alone run 8 per
-    workgroup (20 = two full chunks and a partial one).  Each trial's output
-    must not depend on that choice (bit for bit), and must match the FP64
-    oracle."""
+    20 blocks per call: one partial chunk); 8 trials alone run 1 per
+    workgroup (too few workgroups otherwise).  Each trial's output must not
+    depend on that choice (bit for bit), and must match the FP64 oracle."""
     bw, f0 = 64e6, 1368e6
     dms = np.linspace(0.0, 50.0, 512).astype(np.float32)
     u = make_input(2 * 20 * 2048, 41)

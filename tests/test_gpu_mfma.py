@@ -109,7 +109,7 @@ def test_mfma_full_scale_inputs(device, kind):
 
 def test_mfma_blocks_per_workgroup_choice(device):
     # 512 trials run 16 blocks per workgroup (20 per call: a full chunk and a
-    # partial one), 8 trials alone run 8: each trial's output must not depend
+    # partial one), 8 trials alone run 1: each trial's output must not depend
     # on that choice (bit for bit)
     dms = np.linspace(0.0, 50.0, 512).astype(np.float32)
     u = make_input(2 * 20 * 8192, 43)
