@@ -268,12 +268,12 @@ def other_configs(ms, torch, dev):
     L = 1 << 28
     d_in = random_u16_words(torch, dev, L, 0x5EED0002)
     c2 = {"plain_f32, FFT 2^16 (K 152)": time_engine(ms, torch, ms.Plan(128e6, 100.0, 1336e6, fftpoint=65536,
-                                                                          count=32, form=ms.FORM_PLAIN_F32),
+                                                                          count=256, form=ms.FORM_PLAIN_F32),
                                                      d_in, L, steps=2)}
     for name, form in (("mixed_f16", ms.FORM_FUSED_F16), ("fused_f32", ms.FORM_FUSED_F32)):
         c2[name + ", FFT 2^14 (K 608)"] = time_engine(ms, torch, ms.Plan(128e6, 100.0, 1336e6, fftpoint=16384,
-                                                                         count=64, form=form), d_in, L, steps=2)
-    out["single trial, 128 MHz, DM 100, 2^28 samples"] = c2
+                                                                         count=1024, form=form), d_in, L, steps=2)
+    out["single trial, 128 MHz, DM 100, 2^28 samples, 2^23 samples per call"] = c2
     del d_in
     torch.cuda.empty_cache()
     rng = np.random.Generator(np.random.Philox(0x5EED0004))
