@@ -169,12 +169,15 @@ __global__ __launch_bounds__(64 * kWaves) void k_hankel(const void *__restrict__
     // (batches of kLoadBatch independent loads per thread, issued before any
     // is converted, so they are in flight together)
     // 32-bit slot arithmetic: slot(u) = (s0 - u) mod rx with s0 the slot of
-    // block g0 + gb + kGTile - 1; u < kGTile - 1 + kmax <= rx + kGTile - 1, so
-    // at most two wraps.  Blocks before the stream start (u > g0 + gb +
-    // kGTile - 1) read as zero.
+    // block g0 + gb + kGTile - 1 (a true modulo: with few blocks per call and
+    // a short delay line, u can exceed s0 by several times rx).  Blocks past
+    // the newest one of this call (u < ufirst: they only feed output columns
+    // beyond count) and before the stream start (u > g0 + gb + kGTile - 1)
+    // read as zero.
     constexpr int kLoadBatch = 8;
     const long long top = g0 + gb + kGTile - 1;
     const int s0 = (int)(top % rx);
+    const int ufirst = max(0, gb + kGTile - count);                     // block top - u <= g0 + count - 1
     const int ulim = (int)min(top + 1, (long long)(kGTile - 1 + kmax)); // loaded u < ulim
     for (int base = 0; base < NBIN * U; base += kLoadBatch * 64 * kWaves) {
         RingT raw[kLoadBatch];
@@ -183,9 +186,8 @@ __global__ __launch_bounds__(64 * kWaves) void k_hankel(const void *__restrict__
         for (int r = 0; r < kLoadBatch; r++) {
             const int idx = base + r * 64 * kWaves + tid;
             const int nl = idx % NBIN, u = idx / NBIN;
-            use[r] = idx < NBIN * U && u < ulim;
-            int sl = s0 - (use[r] ? u : 0);
-            sl += sl < 0 ? rx : 0;
+            use[r] = idx < NBIN * U && u >= ufirst && u < ulim;
+            int sl = (s0 - (use[r] ? u : 0)) % rx;
             sl += sl < 0 ? rx : 0;
             raw[r] = ring[(size_t)sl * N + n0 + nl];
         }

@@ -211,6 +211,28 @@ def test_osm_two_pass_fft(device):
     assert rel_l2(y, ref) < FP32_TOL
 
 
+@pytest.mark.parametrize("form", [ms.FORM_FUSED_F32, ms.FORM_FUSED_F16, ms.FORM_HANKEL_F16, ms.FORM_HANKEL_BF16X3])
+def test_osm_method_shared_ring_forms(device, form):
+    # single-segment overlap-save (K = 1, no delay line) for three trials with
+    # a common M = 4096 (FFT 8192), through the shared-ring forms; 4 blocks
+    # per call against a ring of 4 slots also covers the Hankel staging's slot
+    # modulo (a 16-block tile reaches back several times around the ring)
+    bw, f0 = 16e6, 1392e6
+    dms = [0.5, 1.0, 2.0]
+    p = ms.Plan(bw, dms, f0, fftpoint=0, count=4, method=ms.METHOD_OSM, form=form)
+    assert p.M == 4096 and p.K_max == 1
+    u = make_input(3 * 4 * 4096, 22)
+    y = run_engine(p, u, device=device)
+    for i, dm in enumerate(dms):
+        # each trial's own OSM filter at the common M
+        x = oracle.unpack_u16(u).astype(np.complex128)
+        H = oracle.dedisp_params(bw, dm, f0, 4096)
+        ref = oracle.msosm_run(x, 4096, np.zeros(8192, np.int32), H)
+        err = rel_l2(y[i], ref)
+        print(f"OSM form {form} DM {dm}: rel L2 {err:.2e}", flush=True)
+        assert err < TOL[form]
+
+
 def test_config2_fft_size_plain(device):
     # N = 2^16 (the 128 MHz production plan's FFT size) at DM 5: two-pass path
     bw, dm, f0 = 128e6, 5.0, 1336e6
