@@ -122,6 +122,24 @@ def test_multi_dm_vs_fp64(device, form):
         assert err < TOL[form], (i, dm, err)
 
 
+@pytest.mark.parametrize("count,calls", [(1, 8), (3, 5)])
+@pytest.mark.parametrize("form", FORMS + [ms.FORM_FUSED_F16_MFMA])
+def test_short_delay_line_few_blocks(device, form, count, calls):
+    # K from 1 to 3 at FFT 16384 with 1 or 3 blocks per call: a ring of only
+    # K_max - 1 + count slots that every call wraps, and tiles of 16 blocks
+    # (Hankel) or of 16 / 32 blocks per workgroup that a call fills in part
+    bw, f0 = 64e6, 1368e6
+    dms = [0.0, 1.0, 1.5]
+    p = ms.Plan(bw, dms, f0, fftpoint=16384, count=count, form=form)
+    assert p.K_max == 3
+    u = make_input(calls * count * 8192, 31 + count)
+    print(f"short delay line form={form} count={count} calls={calls}", flush=True)
+    y = run_engine(p, u, device=device)
+    for i, dm in enumerate(dms):
+        err = rel_l2(y[i], oracle_out(bw, dm, f0, 8192, u))
+        assert err < TOL.get(form, MIXED_TOL), (dm, err)
+
+
 def test_block_split_invariance(device):
     # 1 call of 16 blocks == 16 calls of 1 block == 4 calls of 4 (bit for bit)
     bw, dm, f0 = 16e6, 10.0, 1392e6
