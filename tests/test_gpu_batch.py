@@ -15,23 +15,37 @@ STREAMS = [
     (32e6, 1384e6, [5.0, 150.0], 4096, 3, 4, ms.FORM_PLAIN_F32, 10000.5),
     (64e6, 1368e6, list(np.linspace(0, 40, 7)), 16384, 2, 3, ms.FORM_FUSED_F16, 64000.25),
 ]
+# the matrix-core and Hankel forms and the single-segment method (FFT size
+# 0: the OSM planner's own) in one graph
+STREAMS_MORE = [
+    (64e6, 1368e6, [0.0, 12.0, 33.0], 16384, 2, 3, ms.FORM_FUSED_F16_MFMA, 50000.5),
+    (32e6, 1384e6, [4.0, 20.0, 45.0, 80.0], 8192, 3, 4, ms.FORM_HANKEL_F16, 9000.75),
+    (16e6, 1392e6, [7.0, 30.0], 4096, 5, 3, ms.FORM_HANKEL_BF16X3, 3333.3),
+    (16e6, 1392e6, [0.5, 2.0], 0, 3, 3, ms.FORM_FUSED_F16, 2500.0, ms.METHOD_OSM),
+]
 
 
-def _inputs(device):
+def _plan(st):
+    bw, f0, dms, fp, count, calls, form, per = st[:8]
+    method = st[8] if len(st) > 8 else ms.METHOD_MSOSM
+    return ms.Plan(bw, dms, f0, fftpoint=fp, count=count, form=form, method=method)
+
+
+def _inputs(device, streams):
     import torch
     out = []
-    for k, (bw, f0, dms, fp, count, calls, form, per) in enumerate(STREAMS):
-        n = count * (fp // 2) * calls
+    for k, st in enumerate(streams):
+        n = st[4] * _plan(st).M * st[5]
         rng = np.random.Generator(np.random.Philox(100 + k))
         u = np.clip(np.round(32768 + 800 * rng.standard_normal(2 * n)), 1, 65535).astype(np.uint16)
         out.append((u, torch.from_numpy(u.view(np.int32)).to(device)))
     return out
 
 
-def _eager(device, k, dev_in, nbins):
+def _eager(device, streams, k, dev_in, nbins):
     import torch
-    bw, f0, dms, fp, count, calls, form, per = STREAMS[k]
-    p = ms.Plan(bw, dms, f0, fftpoint=fp, count=count, form=form)
+    bw, f0, dms, fp, count, calls, form, per = streams[k][:8]
+    p = _plan(streams[k])
     e = ms.Engine(p)
     n = count * p.M
     prof = torch.zeros((len(dms), nbins), dtype=torch.float32, device=device)
@@ -46,22 +60,24 @@ def _eager(device, k, dev_in, nbins):
     return prof.cpu().numpy(), hits.cpu().numpy(), last
 
 
-def test_batch_graph_matches_eager(device):
+@pytest.mark.parametrize("streams", [STREAMS, STREAMS_MORE], ids=["fused_plain", "mfma_hankel_osm"])
+def test_batch_graph_matches_eager(device, streams):
     import torch
     nbins = 64
-    ins = _inputs(device)
+    ins = _inputs(device, streams)
     engines, samples, periods = [], [], []
-    for k, (bw, f0, dms, fp, count, calls, form, per) in enumerate(STREAMS):
-        engines.append(ms.Engine(ms.Plan(bw, dms, f0, fftpoint=fp, count=count, form=form)))
-        samples.append(count * (fp // 2) * calls)
-        periods.append(per)
+    for st in streams:
+        p = _plan(st)
+        engines.append(ms.Engine(p))
+        samples.append(st[4] * p.M * st[5])
+        periods.append(st[7])
     batch = ms.Batch(engines, [d for _, d in ins], samples, periods, nbins=nbins)
-    assert batch.graph_nodes > sum(s[5] for s in STREAMS)
-    ref = [_eager(device, k, ins[k][1], nbins) for k in range(len(STREAMS))]
+    assert batch.graph_nodes > sum(s[5] for s in streams)
+    ref = [_eager(device, streams, k, ins[k][1], nbins) for k in range(len(streams))]
     for replay in range(3):
         batch.launch()
         batch.synchronize()
-        for k in range(len(STREAMS)):
+        for k in range(len(streams)):
             prof, hits = batch.profiles(k)
             rprof, rhits, rlast = ref[k]
             np.testing.assert_array_equal(hits, rhits)
