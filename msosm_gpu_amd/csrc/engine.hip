@@ -115,6 +115,8 @@ struct msosm_engine {
     void *ybuf = nullptr;           // [ndm][count][2M] products (half2 or float2)
     int hankel_mode = -1;
     int kmax = 0;                   // largest delay + 1 over all trials
+    int hankel_tpg = 0;             // DM tiles per Hankel workgroup (grid z groups)
+    int hankel_kspan = 0;           // widest delay band one Hankel workgroup stages
     long long g0 = 0;               // global index of the next block
     bool have_output = false;
     // host input: copies run on copy_stream into alternate halves of in_stage;
@@ -354,6 +356,8 @@ int create_engine(const msosm::Plan &plan, void *stream, msosm_engine **out)
                                   "Hankel forms hold up to %d delay segments per bin in LDS (plan has %d); use a "
                                   "larger FFT size or a fused form",
                                   msosm::hankel_max_kmax(e->hankel_mode), km));
+        msosm::hankel_groups(e->hankel_mode, p.delay.data(), (int)p.num_dms, (int)N, km, &e->hankel_tpg,
+                             &e->hankel_kspan);
     }
     int ndev = 0;
     hipError_t herr = hipGetDeviceCount(&ndev);
@@ -542,7 +546,8 @@ int filter_hankel(msosm_engine *e, const uint32_t *cur)
     {
         StageScope st(e, MSOSM_STAGE_MULTIPLY);
         HIP_TRY(msosm::launch_hankel(e->xring, f16 ? (const void *)e->H16 : (const void *)e->H32, e->delay16, e->ybuf,
-                                     e->hankel_mode, e->log2n, count, ndm, e->g0, e->rx, e->kmax, s));
+                                     e->hankel_mode, e->log2n, count, ndm, e->g0, e->rx, e->kmax, e->hankel_tpg,
+                                     e->hankel_kspan, s));
     }
     {
         StageScope st(e, MSOSM_STAGE_FFT_INV);
@@ -918,7 +923,7 @@ int msosm_k_hankel_delay_gemm(const void *ring, const void *H, const uint16_t *d
     if (kmax > msosm::hankel_max_kmax(mode))
         return set_error(MSOSM_ERR_UNSUPPORTED, "hankel_delay_gemm: kmax %d above %d", kmax,
                          msosm::hankel_max_kmax(mode));
-    HIP_TRY(msosm::launch_hankel(ring, H, delay, Y, mode, log2n, count, num_dms, g0, rx, kmax,
+    HIP_TRY(msosm::launch_hankel(ring, H, delay, Y, mode, log2n, count, num_dms, g0, rx, kmax, 0, kmax,
                                  (hipStream_t)stream));
     return MSOSM_OK;
 }

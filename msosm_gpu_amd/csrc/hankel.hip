@@ -36,7 +36,9 @@
 #include <algorithm>
 #include <climits>
 #include <cstdint>
+#include <cstdlib>
 #include <type_traits>
+#include <vector>
 
 #include "fft_device.h"
 #include "kernels.h"
@@ -50,6 +52,7 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kDmTile = 16; // trials per MFMA tile (rows)
+constexpr size_t kHankelThreePerCu = 160 * 1024 / 3;
 constexpr int kGTile = 16;  // output blocks per workgroup (columns)
 constexpr int kWaves = 4;
 
@@ -135,7 +138,7 @@ template <int MODE>
 __global__ __launch_bounds__(64 * kWaves) void k_hankel(const void *__restrict__ ring_, const void *__restrict__ H_,
                                                         const uint16_t *__restrict__ delay, void *__restrict__ Y_,
                                                         int N, int count, int ndm, long long g0, int rx, int kmax,
-                                                        int U)
+                                                        int U, int tpg)
 {
     using Md = HankelMode<MODE>;
     using RingT = typename Md::RingT;
@@ -163,6 +166,27 @@ __global__ __launch_bounds__(64 * kWaves) void k_hankel(const void *__restrict__
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
 
+    // This workgroup's trials: DM tiles [tg0, tg1) (grid z), and the band of
+    // delays [lo_w, hi_w] they take at its bins; only ring columns u >= lo_w
+    // are staged (LDS column u' = u - lo_w, U >= hi_w - lo_w + 31 columns).
+    const int ntiles = (ndm + kDmTile - 1) / kDmTile;
+    const int tg0 = blockIdx.z * tpg, tg1 = min(ntiles, tg0 + tpg);
+    int lo_w = INT_MAX;
+    {
+        const int dm0 = tg0 * kDmTile, ntr = min(ndm, tg1 * kDmTile) - dm0;
+        for (int idx = tid; idx < ntr * NBIN; idx += 64 * kWaves)
+            lo_w = min(lo_w, (int)delay[(long long)(dm0 + idx / NBIN) * N + n0 + idx % NBIN]);
+#pragma unroll
+        for (int m = 1; m < 64; m <<= 1)
+            lo_w = min(lo_w, __shfl_xor(lo_w, m));
+        if (lane == 0)
+            dd[wave] = lo_w; // dd is free until the first tile
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < kWaves; i++)
+            lo_w = min(lo_w, dd[i]);
+    }
+
     // ring columns of this workgroup's bins: u -> absolute block
     // a = g0 + gb + kGTile - 1 - u; zero before the stream start and past the
     // oldest block any (g, k) of the tile uses
@@ -185,7 +209,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_hankel(const void *__restrict__
 #pragma unroll
         for (int r = 0; r < kLoadBatch; r++) {
             const int idx = base + r * 64 * kWaves + tid;
-            const int nl = idx % NBIN, u = idx / NBIN;
+            const int nl = idx % NBIN, u = lo_w + idx / NBIN;
             use[r] = idx < NBIN * U && u >= ufirst && u < ulim;
             int sl = (s0 - (use[r] ? u : 0)) % rx;
             sl += sl < 0 ? rx : 0;
@@ -196,7 +220,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_hankel(const void *__restrict__
             const int idx = base + r * 64 * kWaves + tid;
             if (idx >= NBIN * U)
                 break;
-            const int nl = idx % NBIN, u = idx / NBIN;
+            const int nl = idx % NBIN, u = idx / NBIN; // LDS column
             const float2 x = use[r] ? Md::load(&raw[r]) : make_float2(0.f, 0.f);
             Op rh, rl, ih, il;
             split<MODE>(x.x, rh, rl);
@@ -214,7 +238,6 @@ __global__ __launch_bounds__(64 * kWaves) void k_hankel(const void *__restrict__
     const int q = lane >> 4;        // k group: 0,1 real rows, 2,3 imaginary rows
     const int kq = 8 * (q & 1);
     const bool imag_half = q >= 2;
-    const int ntiles = (ndm + kDmTile - 1) / kDmTile;
     // one DM tile's delays and filter values, kDmTile * NBIN entries: at most
     // PER per thread, loaded one tile ahead so the loads overlap the MFMA
     // work of the current tile
@@ -231,8 +254,8 @@ __global__ __launch_bounds__(64 * kWaves) void k_hankel(const void *__restrict__
             hr[r] = H[(long long)dm * N + n0 + nl];
         }
     };
-    load_tables(0);
-    for (int tile = 0; tile < ntiles; tile++) {
+    load_tables(tg0);
+    for (int tile = tg0; tile < tg1; tile++) {
         __syncthreads(); // z ready (first tile); tables and staging free
 #pragma unroll
         for (int r = 0; r < PER; r++) {
@@ -244,7 +267,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_hankel(const void *__restrict__
             }
         }
         __syncthreads();
-        if (tile + 1 < ntiles)
+        if (tile + 1 < tg1)
             load_tables(tile + 1);
         for (int nl = wave; nl < NBIN; nl += kWaves) {
             const int d = dd[row * NBIN + nl];
@@ -261,8 +284,8 @@ __global__ __launch_bounds__(64 * kWaves) void k_hankel(const void *__restrict__
             split<MODE>(imag_half ? -h.y : h.x, ar_h, ar_l);
             split<MODE>(imag_half ? h.x : h.y, ai_h, ai_l);
             const Op zero = to_op<Op>(0.f);
-            const Op *zb = z + ((imag_half ? 1 : 0) * NBIN + nl) * U + (kGTile - 1 - row) + kq;
-            const Op *zl = z + ((imag_half ? 3 : 2) * NBIN + nl) * U + (kGTile - 1 - row) + kq;
+            const Op *zb = z + ((imag_half ? 1 : 0) * NBIN + nl) * U + (kGTile - 1 - row) + kq - lo_w;
+            const Op *zl = z + ((imag_half ? 3 : 2) * NBIN + nl) * U + (kGTile - 1 - row) + kq - lo_w;
             for (int k0 = lo; k0 <= hi; k0 += 16) { // wave-uniform bounds
                 OpV a_r, a_i, b;
 #pragma unroll
@@ -436,12 +459,17 @@ size_t hankel_lds_bytes(int U)
 
 template <int MODE>
 hipError_t hankel_launch(const void *ring, const void *H, const uint16_t *delay, void *Y, int N, int count, int ndm,
-                         long long g0, int rx, int kmax, hipStream_t s)
+                         long long g0, int rx, int kmax, int tpg, int kspan, hipStream_t s)
 {
     using Md = HankelMode<MODE>;
+    const int ntiles = (ndm + kDmTile - 1) / kDmTile;
+    if (tpg <= 0 || tpg >= ntiles || kspan <= 0 || kspan > kmax) {
+        tpg = ntiles; // one group: the band is within [0, kmax)
+        kspan = kmax;
+    }
     if (N % Md::NBIN != 0)
         return hipErrorInvalidValue;
-    const int U = kGTile - 1 + kmax + 16;
+    const int U = kGTile - 1 + kspan + 16;
     const size_t lds = hankel_lds_bytes<MODE>(U);
     if (lds > kHankelMaxLds)
         return hipErrorInvalidValue;
@@ -453,9 +481,9 @@ hipError_t hankel_launch(const void *ring, const void *H, const uint16_t *delay,
             return e;
         attr_set = true;
     }
-    dim3 grid(N / Md::NBIN, (count + kGTile - 1) / kGTile);
+    dim3 grid(N / Md::NBIN, (count + kGTile - 1) / kGTile, (ntiles + tpg - 1) / tpg);
     hipLaunchKernelGGL(k_hankel<MODE>, grid, dim3(64 * kWaves), lds, s, ring, H, delay, Y, N, count, ndm, g0, rx,
-                       kmax, U);
+                       kmax, U, tpg);
     return hipGetLastError();
 }
 
@@ -512,14 +540,69 @@ int hankel_max_kmax(int mode)
 
 int hankel_bins_per_wg(int mode) { return mode == HANKEL_F16 ? HankelMode<HANKEL_F16>::NBIN : HankelMode<HANKEL_BF16>::NBIN; }
 
+static size_t hankel_lds_of(int mode, int U)
+{
+    return mode == HANKEL_F16 ? hankel_lds_bytes<HANKEL_F16>(U)
+           : mode == HANKEL_BF16 ? hankel_lds_bytes<HANKEL_BF16>(U)
+                                 : hankel_lds_bytes<HANKEL_BF16X3>(U);
+}
+
+void hankel_groups(int mode, const int32_t *delay, int ndm, int N, int kmax, int *tpg, int *kspan)
+{
+    const int nbin = hankel_bins_per_wg(mode), ngrp = N / nbin;
+    const int ntiles = (ndm + kDmTile - 1) / kDmTile;
+    // per (trial, bin group): lowest and highest delay
+    std::vector<int> lo((size_t)ndm * ngrp, INT_MAX), hi((size_t)ndm * ngrp, 0);
+    for (int d = 0; d < ndm; d++)
+        for (int n = 0; n < N; n++) {
+            const size_t i = (size_t)d * ngrp + n / nbin;
+            const int v = delay[(size_t)d * N + n];
+            lo[i] = std::min(lo[i], v);
+            hi[i] = std::max(hi[i], v);
+        }
+    // widest band (hi - lo + 1) over all workgroups when groups hold t tiles
+    auto span = [&](int t) {
+        int w = 1;
+        for (int g0 = 0; g0 < ntiles; g0 += t)
+            for (int b = 0; b < ngrp; b++) {
+                int l = INT_MAX, h = 0;
+                for (int d = g0 * kDmTile; d < std::min(ndm, (g0 + t) * kDmTile); d++) {
+                    l = std::min(l, lo[(size_t)d * ngrp + b]);
+                    h = std::max(h, hi[(size_t)d * ngrp + b]);
+                }
+                w = std::max(w, h - l + 1);
+            }
+        return std::min(w, kmax);
+    };
+    // the largest group (fewest repeated ring columns) whose LDS leaves room
+    // for three workgroups per CU (the staged output alone takes 34 KB for
+    // FP16); MSOSM_HANKEL_TPG overrides
+    int t = ntiles, w = kmax;
+    const char *env = getenv("MSOSM_HANKEL_TPG");
+    if (env && atoi(env) > 0) {
+        t = std::min(atoi(env), ntiles);
+        w = span(t);
+    } else {
+        for (;;) {
+            w = span(t);
+            if (hankel_lds_of(mode, kGTile - 1 + w + 16) <= kHankelThreePerCu || t == 1)
+                break;
+            t = (t + 1) / 2;
+        }
+    }
+    *tpg = t;
+    *kspan = w;
+}
+
 hipError_t launch_hankel(const void *ring, const void *H, const uint16_t *delay, void *Y, int mode, int log2n,
-                         int count, int ndm, long long g0, int rx, int kmax, hipStream_t s)
+                         int count, int ndm, long long g0, int rx, int kmax, int tpg, int kspan, hipStream_t s)
 {
     const int N = 1 << log2n;
     switch (mode) {
-    case HANKEL_F16: return hankel_launch<HANKEL_F16>(ring, H, delay, Y, N, count, ndm, g0, rx, kmax, s);
-    case HANKEL_BF16: return hankel_launch<HANKEL_BF16>(ring, H, delay, Y, N, count, ndm, g0, rx, kmax, s);
-    case HANKEL_BF16X3: return hankel_launch<HANKEL_BF16X3>(ring, H, delay, Y, N, count, ndm, g0, rx, kmax, s);
+    case HANKEL_F16: return hankel_launch<HANKEL_F16>(ring, H, delay, Y, N, count, ndm, g0, rx, kmax, tpg, kspan, s);
+    case HANKEL_BF16: return hankel_launch<HANKEL_BF16>(ring, H, delay, Y, N, count, ndm, g0, rx, kmax, tpg, kspan, s);
+    case HANKEL_BF16X3:
+        return hankel_launch<HANKEL_BF16X3>(ring, H, delay, Y, N, count, ndm, g0, rx, kmax, tpg, kspan, s);
     default: return hipErrorInvalidValue;
     }
 }

@@ -62,7 +62,12 @@ constexpr int HANKEL_BF16 = 1;
 constexpr int HANKEL_BF16X3 = 2;
 constexpr size_t kHankelMaxLds = 80 * 1024;
 hipError_t launch_hankel(const void *ring, const void *H, const uint16_t *delay, void *Y, int mode, int log2n,
-                         int count, int ndm, long long g0, int rx, int kmax, hipStream_t s);
+                         int count, int ndm, long long g0, int rx, int kmax, int tpg, int kspan, hipStream_t s);
+// Trial grouping of launch_hankel from the host delay table [ndm][N]: tpg DM
+// tiles of 16 trials per workgroup (grid z) and the widest delay band kspan
+// any workgroup stages; tpg <= 0 (or kspan = kmax, tpg = all tiles) stages
+// the whole delay line.
+void hankel_groups(int mode, const int32_t *delay, int ndm, int N, int kmax, int *tpg, int *kspan);
 int hankel_max_kmax(int mode);
 int hankel_bins_per_wg(int mode);
 // Inverse FFT of dense rows Y[unit][2^log2n] (FP16 or FP32 complex), keep the
