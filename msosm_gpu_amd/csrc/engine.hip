@@ -117,6 +117,8 @@ struct msosm_engine {
     int kmax = 0;                   // largest delay + 1 over all trials
     int hankel_tpg = 0;             // DM tiles per Hankel workgroup (grid z groups)
     int hankel_kspan = 0;           // widest delay band one Hankel workgroup stages
+    std::vector<int> hankel_lo_h;   // lowest delay per (trial group, bin group)
+    int *hankel_lo = nullptr;       // its device copy
     long long g0 = 0;               // global index of the next block
     bool have_output = false;
     // host input: copies run on copy_stream into alternate halves of in_stage;
@@ -213,7 +215,7 @@ void free_engine(msosm_engine *e)
             (void)hipEventDestroy(ev);
     void *bufs[] = {e->in_stage, e->H32,  e->H16,     e->delay32,  e->delay16, e->delay_pk, e->Hp,
                     e->dpk_mfma, e->mfma_tab, e->out, e->out_u16, e->inbuf,
-                    e->fftblk, e->ring, e->ifftblk, e->fft_scratch, e->tail, e->xring, e->ybuf};
+                    e->fftblk, e->ring, e->ifftblk, e->fft_scratch, e->tail, e->xring, e->ybuf, e->hankel_lo};
     for (void *b : bufs)
         if (b)
             (void)hipFree(b);
@@ -357,7 +359,7 @@ int create_engine(const msosm::Plan &plan, void *stream, msosm_engine **out)
                                   "larger FFT size or a fused form",
                                   msosm::hankel_max_kmax(e->hankel_mode), km));
         msosm::hankel_groups(e->hankel_mode, p.delay.data(), (int)p.num_dms, (int)N, km, &e->hankel_tpg,
-                             &e->hankel_kspan);
+                             &e->hankel_kspan, &e->hankel_lo_h);
     }
     int ndev = 0;
     hipError_t herr = hipGetDeviceCount(&ndev);
@@ -410,6 +412,9 @@ int create_engine(const msosm::Plan &plan, void *stream, msosm_engine **out)
                 ALLOC(y, (size_t)ndm * count * N);
                 e->ybuf = y;
             }
+            ALLOC(e->hankel_lo, e->hankel_lo_h.size());
+            HIP_TRY(hipMemcpy(e->hankel_lo, e->hankel_lo_h.data(), sizeof(int) * e->hankel_lo_h.size(),
+                              hipMemcpyHostToDevice));
         }
         ALLOC(e->delay16, (size_t)ndm * N);
         ALLOC(e->tail, 2 * (size_t)M); // two halves, alternating by call
@@ -547,7 +552,7 @@ int filter_hankel(msosm_engine *e, const uint32_t *cur)
         StageScope st(e, MSOSM_STAGE_MULTIPLY);
         HIP_TRY(msosm::launch_hankel(e->xring, f16 ? (const void *)e->H16 : (const void *)e->H32, e->delay16, e->ybuf,
                                      e->hankel_mode, e->log2n, count, ndm, e->g0, e->rx, e->kmax, e->hankel_tpg,
-                                     e->hankel_kspan, s));
+                                     e->hankel_kspan, e->hankel_lo, s));
     }
     {
         StageScope st(e, MSOSM_STAGE_FFT_INV);
@@ -923,7 +928,7 @@ int msosm_k_hankel_delay_gemm(const void *ring, const void *H, const uint16_t *d
     if (kmax > msosm::hankel_max_kmax(mode))
         return set_error(MSOSM_ERR_UNSUPPORTED, "hankel_delay_gemm: kmax %d above %d", kmax,
                          msosm::hankel_max_kmax(mode));
-    HIP_TRY(msosm::launch_hankel(ring, H, delay, Y, mode, log2n, count, num_dms, g0, rx, kmax, 0, kmax,
+    HIP_TRY(msosm::launch_hankel(ring, H, delay, Y, mode, log2n, count, num_dms, g0, rx, kmax, 0, kmax, nullptr,
                                  (hipStream_t)stream));
     return MSOSM_OK;
 }

@@ -138,7 +138,7 @@ template <int MODE>
 __global__ __launch_bounds__(64 * kWaves) void k_hankel(const void *__restrict__ ring_, const void *__restrict__ H_,
                                                         const uint16_t *__restrict__ delay, void *__restrict__ Y_,
                                                         int N, int count, int ndm, long long g0, int rx, int kmax,
-                                                        int U, int tpg)
+                                                        int U, int tpg, const int *__restrict__ band_lo)
 {
     using Md = HankelMode<MODE>;
     using RingT = typename Md::RingT;
@@ -168,24 +168,11 @@ __global__ __launch_bounds__(64 * kWaves) void k_hankel(const void *__restrict__
 
     // This workgroup's trials: DM tiles [tg0, tg1) (grid z), and the band of
     // delays [lo_w, hi_w] they take at its bins; only ring columns u >= lo_w
-    // are staged (LDS column u' = u - lo_w, U >= hi_w - lo_w + 31 columns).
+    // are staged (LDS column u' = u - lo_w, U >= hi_w - lo_w + 31 columns)
     const int ntiles = (ndm + kDmTile - 1) / kDmTile;
     const int tg0 = blockIdx.z * tpg, tg1 = min(ntiles, tg0 + tpg);
-    int lo_w = INT_MAX;
-    {
-        const int dm0 = tg0 * kDmTile, ntr = min(ndm, tg1 * kDmTile) - dm0;
-        for (int idx = tid; idx < ntr * NBIN; idx += 64 * kWaves)
-            lo_w = min(lo_w, (int)delay[(long long)(dm0 + idx / NBIN) * N + n0 + idx % NBIN]);
-#pragma unroll
-        for (int m = 1; m < 64; m <<= 1)
-            lo_w = min(lo_w, __shfl_xor(lo_w, m));
-        if (lane == 0)
-            dd[wave] = lo_w; // dd is free until the first tile
-        __syncthreads();
-#pragma unroll
-        for (int i = 0; i < kWaves; i++)
-            lo_w = min(lo_w, dd[i]);
-    }
+    // (band_lo[group][bin group] from the host's delay table; none: 0)
+    const int lo_w = band_lo ? band_lo[(size_t)blockIdx.z * gridDim.x + blockIdx.x] : 0;
 
     // ring columns of this workgroup's bins: u -> absolute block
     // a = g0 + gb + kGTile - 1 - u; zero before the stream start and past the
@@ -459,13 +446,14 @@ size_t hankel_lds_bytes(int U)
 
 template <int MODE>
 hipError_t hankel_launch(const void *ring, const void *H, const uint16_t *delay, void *Y, int N, int count, int ndm,
-                         long long g0, int rx, int kmax, int tpg, int kspan, hipStream_t s)
+                         long long g0, int rx, int kmax, int tpg, int kspan, const int *band_lo, hipStream_t s)
 {
     using Md = HankelMode<MODE>;
     const int ntiles = (ndm + kDmTile - 1) / kDmTile;
-    if (tpg <= 0 || tpg >= ntiles || kspan <= 0 || kspan > kmax) {
-        tpg = ntiles; // one group: the band is within [0, kmax)
+    if (!band_lo || tpg <= 0 || tpg >= ntiles || kspan <= 0 || kspan > kmax) {
+        tpg = ntiles; // one group staging the whole line [0, kmax)
         kspan = kmax;
+        band_lo = nullptr;
     }
     if (N % Md::NBIN != 0)
         return hipErrorInvalidValue;
@@ -483,7 +471,7 @@ hipError_t hankel_launch(const void *ring, const void *H, const uint16_t *delay,
     }
     dim3 grid(N / Md::NBIN, (count + kGTile - 1) / kGTile, (ntiles + tpg - 1) / tpg);
     hipLaunchKernelGGL(k_hankel<MODE>, grid, dim3(64 * kWaves), lds, s, ring, H, delay, Y, N, count, ndm, g0, rx,
-                       kmax, U, tpg);
+                       kmax, U, tpg, band_lo);
     return hipGetLastError();
 }
 
@@ -547,7 +535,8 @@ static size_t hankel_lds_of(int mode, int U)
                                  : hankel_lds_bytes<HANKEL_BF16X3>(U);
 }
 
-void hankel_groups(int mode, const int32_t *delay, int ndm, int N, int kmax, int *tpg, int *kspan)
+void hankel_groups(int mode, const int32_t *delay, int ndm, int N, int kmax, int *tpg, int *kspan,
+                   std::vector<int> *band_lo)
 {
     const int nbin = hankel_bins_per_wg(mode), ngrp = N / nbin;
     const int ntiles = (ndm + kDmTile - 1) / kDmTile;
@@ -592,17 +581,26 @@ void hankel_groups(int mode, const int32_t *delay, int ndm, int N, int kmax, int
     }
     *tpg = t;
     *kspan = w;
+    // lowest delay per (group, bin group), the first column a workgroup stages
+    band_lo->assign((size_t)((ntiles + t - 1) / t) * ngrp, INT_MAX);
+    for (int g = 0; g * t < ntiles; g++)
+        for (int d = g * t * kDmTile; d < std::min(ndm, (g + 1) * t * kDmTile); d++)
+            for (int b = 0; b < ngrp; b++)
+                (*band_lo)[(size_t)g * ngrp + b] = std::min((*band_lo)[(size_t)g * ngrp + b], lo[(size_t)d * ngrp + b]);
 }
 
 hipError_t launch_hankel(const void *ring, const void *H, const uint16_t *delay, void *Y, int mode, int log2n,
-                         int count, int ndm, long long g0, int rx, int kmax, int tpg, int kspan, hipStream_t s)
+                         int count, int ndm, long long g0, int rx, int kmax, int tpg, int kspan, const int *band_lo,
+                         hipStream_t s)
 {
     const int N = 1 << log2n;
     switch (mode) {
-    case HANKEL_F16: return hankel_launch<HANKEL_F16>(ring, H, delay, Y, N, count, ndm, g0, rx, kmax, tpg, kspan, s);
-    case HANKEL_BF16: return hankel_launch<HANKEL_BF16>(ring, H, delay, Y, N, count, ndm, g0, rx, kmax, tpg, kspan, s);
+    case HANKEL_F16:
+        return hankel_launch<HANKEL_F16>(ring, H, delay, Y, N, count, ndm, g0, rx, kmax, tpg, kspan, band_lo, s);
+    case HANKEL_BF16:
+        return hankel_launch<HANKEL_BF16>(ring, H, delay, Y, N, count, ndm, g0, rx, kmax, tpg, kspan, band_lo, s);
     case HANKEL_BF16X3:
-        return hankel_launch<HANKEL_BF16X3>(ring, H, delay, Y, N, count, ndm, g0, rx, kmax, tpg, kspan, s);
+        return hankel_launch<HANKEL_BF16X3>(ring, H, delay, Y, N, count, ndm, g0, rx, kmax, tpg, kspan, band_lo, s);
     default: return hipErrorInvalidValue;
     }
 }

@@ -4,6 +4,7 @@
 #include <hip/hip_fp16.h>
 #include <cstddef>
 #include <cstdint>
+#include <vector>
 
 namespace msosm {
 
@@ -62,12 +63,14 @@ constexpr int HANKEL_BF16 = 1;
 constexpr int HANKEL_BF16X3 = 2;
 constexpr size_t kHankelMaxLds = 80 * 1024;
 hipError_t launch_hankel(const void *ring, const void *H, const uint16_t *delay, void *Y, int mode, int log2n,
-                         int count, int ndm, long long g0, int rx, int kmax, int tpg, int kspan, hipStream_t s);
+                         int count, int ndm, long long g0, int rx, int kmax, int tpg, int kspan, const int *band_lo,
+                         hipStream_t s);
 // Trial grouping of launch_hankel from the host delay table [ndm][N]: tpg DM
-// tiles of 16 trials per workgroup (grid z) and the widest delay band kspan
-// any workgroup stages; tpg <= 0 (or kspan = kmax, tpg = all tiles) stages
-// the whole delay line.
-void hankel_groups(int mode, const int32_t *delay, int ndm, int N, int kmax, int *tpg, int *kspan);
+// tiles of 16 trials per workgroup (grid z), the widest delay band kspan any
+// workgroup stages, and band_lo[group][bin group] its lowest delay (a device
+// copy goes to launch_hankel).  band_lo null stages the whole delay line.
+void hankel_groups(int mode, const int32_t *delay, int ndm, int N, int kmax, int *tpg, int *kspan,
+                   std::vector<int> *band_lo);
 int hankel_max_kmax(int mode);
 int hankel_bins_per_wg(int mode);
 // Inverse FFT of dense rows Y[unit][2^log2n] (FP16 or FP32 complex), keep the
