@@ -176,6 +176,17 @@ int msosm_k_discard_batch(const float *a, float *b, int M, int count, int num_dm
  * forward, +1 inverse (unnormalised).  in and out must not overlap. */
 int msosm_k_fft_c2c(const float *in, float *out, size_t n, size_t batch, size_t idist, size_t odist,
                     int direction, void *stream);
+/* As msosm_k_fft_c2c, allocating nothing.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under graph
+ * capture, serving loops, harnesses that forbid hipMalloc).  Sizes above
+ * 2^14 points use a fixed per-device scratch array of 2^22 complex samples
+ * (32 MiB, held by each GPU the library runs on until the process ends)
+ * instead of a per-call buffer: batch * n above 2^22 is rejected with
+ * MSOSM_ERR_ARG.  The stream must belong to the current device.  Calls that
+ * overlap on one GPU share the array and race; the array needs no clearing
+ * between calls. */
+int msosm_k_fft_c2c_noalloc(const float *in, float *out, size_t n, size_t batch, size_t idist, size_t odist,
+                            int direction, void *stream);
 /* The multiply-delay of ComplexMultiplyDelay_concurrent_batch (:23,
  * gpu_kernel.cu:160-197) over a shared spectra ring, as per-bin complex GEMMs
  * on the matrix cores: Y[dm][b][n] = H[dm][n] * ring[(g0 + b - d[dm][n]) mod rx][n]

@@ -18,6 +18,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libmsosm.so")
 INCLUDE_DIR = os.path.join(os.path.dirname(HERE), "include")
 
 OK = 0
+ERR_ARG, ERR_HIP, ERR_STATE, ERR_UNSUPPORTED, ERR_ALLOC = 1, 2, 3, 4, 5
 METHOD_MSOSM, METHOD_OSM = 0, 1
 CHIRP_REF_F32W, CHIRP_EXACT_F64 = 0, 1
 FORM_PLAIN_F32, FORM_FUSED_F32, FORM_FUSED_F16 = 0, 1, 2
@@ -92,6 +93,7 @@ SIGNATURES = {
     "msosm_k_multiply_delay_batch": (_i, [_vp, _vp, _i, _vp, _vp, _i, _i, _i, _i, _vp]),
     "msosm_k_discard_batch": (_i, [_vp, _vp, _i, _i, _i, _vp]),
     "msosm_k_fft_c2c": (_i, [_vp, _vp, _sz, _sz, _sz, _sz, _i, _vp]),
+    "msosm_k_fft_c2c_noalloc": (_i, [_vp, _vp, _sz, _sz, _sz, _sz, _i, _vp]),
     "msosm_host_alloc": (_i, [_sz, ctypes.POINTER(_vp)]),
     "msosm_refsim_info": (_i, [_f, _f, _f, _f, ctypes.POINTER(_ul), ctypes.POINTER(_ul), ctypes.POINTER(_ul),
                                ctypes.POINTER(_ul)]),

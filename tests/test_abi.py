@@ -62,3 +62,15 @@ def test_engine_rejects_a_ring_beyond_32bit_offsets():
     p = ms.Plan(64e6, [100.0], 1368e6, fftpoint=16384, count=40000, form=ms.FORM_FUSED_F32)
     with pytest.raises(ms.MsosmError, match="below 4 GiB"):
         ms.Engine(p)
+
+
+def test_noalloc_fft_refuses_oversized_scratch_on_the_host():
+    # checked before any device call: batch * n beyond the fixed 2^22-sample
+    # scratch, bad sizes and bad directions are argument errors
+    L = ms.lib()
+    p = ctypes.c_void_p(0x1000)
+    assert L.msosm_k_fft_c2c_noalloc(p, p, 1 << 21, 3, 1 << 21, 1 << 21, -1, None) == ms.ERR_ARG
+    assert b"exceeds the fixed scratch" in L.msosm_last_error()
+    assert L.msosm_k_fft_c2c_noalloc(p, p, 1 << 23, 1, 1 << 23, 1 << 23, 1, None) == ms.ERR_ARG
+    assert L.msosm_k_fft_c2c_noalloc(p, p, 3000, 1, 3000, 3000, 1, None) == ms.ERR_ARG
+    assert L.msosm_k_fft_c2c_noalloc(p, p, 1024, 1, 1024, 1024, 0, None) == ms.ERR_ARG
