@@ -26,6 +26,18 @@ static_assert(sizeof(uint16_pair) == 4, "uint16_pair must be 4 bytes");
 constexpr auto pi = M_PI; // globals.h:29
 inline bool verbose = false; // globals.cpp:3
 
+// Named profiler ranges (the reference's NVTX PUSH_RANGE / POP_RANGE,
+// gpu/include/gpu_common.cuh:16-39): rocTX ranges on the calling thread,
+// shown by rocprofv3 --marker-trace.  The colour id has no rocTX counterpart.
+#ifndef PUSH_RANGE
+#define PUSH_RANGE(name, cid)                                                                                    \
+    {                                                                                                            \
+        (void)(cid);                                                                                             \
+        (void)msosm_range_push(name);                                                                            \
+    }
+#define POP_RANGE (void)msosm_range_pop();
+#endif
+
 namespace msosm_facade {
 
 // The reference prints and exits on any CUDA/cuFFT failure

@@ -52,7 +52,10 @@ extern "C" {
 /* (Y_n = A_n B_n, A_n one H per row at the segment delay, B_n the Hankel  */
 /* block of ring spectra), then a separate inverse FFT of the products:     */
 #define MSOSM_FORM_HANKEL_F16 3    /* FP16 operands, FP32 accumulate         */
-#define MSOSM_FORM_HANKEL_BF16 4   /* BF16 operands (single), FP32 accumulate */
+#define MSOSM_FORM_HANKEL_BF16 4   /* BF16 operands (single), FP32 accumulate:*/
+                                   /* misses the 1e-3 mixed gate, so engines */
+                                   /* refuse it; msosm_k_hankel_delay_gemm   */
+                                   /* still takes it (GEMM studies)          */
 #define MSOSM_FORM_HANKEL_BF16X3 5 /* BF16 hi+lo split operands, 3 products  */
 /* As FUSED_F16 with the inverse FFT's DFT stages on the matrix cores (FP16 */
 /* data entering each radix pass, FP32 accumulate and twiddles); FFT size   */
@@ -140,6 +143,40 @@ int msosm_engine_synchronize(msosm_engine *eng);
 int msosm_engine_reset(msosm_engine *eng);
 /* The HIP stream the engine launches on (void* = hipStream_t). */
 int msosm_engine_stream(msosm_engine *eng, void **hip_stream);
+/* Stream state for checkpoint/resume: the history the next filter call
+ * reads -- the overlap tail of the input (the head of the reference's input
+ * double buffer, msosm_gpu_batch.cpp:80), the delay ring (products in the
+ * reference-structured form, shared spectra otherwise) and the ring cursor
+ * (current_index, msosm_gpu_batch.cpp:86,101-108) -- as one byte image of
+ * msosm_engine_state_size bytes.  Export waits for the engine's stream; an
+ * image imports only into an engine of the same plan (trials, FFT size,
+ * blocks per call, method, chirp mode and form), which then continues
+ * exactly as the exporting engine would.  where: 0 host, 1 device memory. */
+int msosm_engine_state_size(msosm_engine *eng, size_t *bytes);
+int msosm_engine_state_export(msosm_engine *eng, void *dst, size_t bytes, int where);
+int msosm_engine_state_import(msosm_engine *eng, const void *src, size_t bytes, int where);
+/* Named rocTX range on the calling thread (the reference's PUSH_RANGE /
+ * POP_RANGE NVTX macros, gpu/include/gpu_common.cuh:16-39).  The library's
+ * own work entry points (filter, output, fold, batch launch, state copies)
+ * open ranges named after themselves. */
+int msosm_range_push(const char *name);
+int msosm_range_pop(void);
+/* Launch-shape requests of an engine (no reference counterpart; cuFFT picks
+ * its own plans).  0 in a field = the built-in rule.
+ *   inv_blocks_per_wg: consecutive blocks of one trial that one workgroup of
+ *     the inverse transform runs (fused, matrix-core and dense inverses at
+ *     4096..16384 points); rule: the largest of 32, 16, ... that leaves at
+ *     least 512 workgroups.
+ *   hankel_tiles_per_group: 16-trial tiles that share one Hankel GEMM
+ *     workgroup's staged delay band; rule: the largest whose band leaves LDS
+ *     for three workgroups per CU (reduced until the band fits the LDS).
+ * get_tuning reports the requests, with the Hankel group size in effect. */
+typedef struct msosm_tuning {
+    int inv_blocks_per_wg;
+    int hankel_tiles_per_group;
+} msosm_tuning;
+int msosm_engine_set_tuning(msosm_engine *eng, const msosm_tuning *tuning);
+int msosm_engine_get_tuning(msosm_engine *eng, msosm_tuning *tuning);
 /* Per-stage timing with HIP events on the engine stream (0 = off; the
  * host-to-device copy of host input is timed on the engine's copy stream).
  * Stage ids: MSOSM_STAGE_*.  times_ms[i] = summed kernel time of stage i,

@@ -63,6 +63,13 @@ class PlanInfo(ctypes.Structure):
     ]
 
 
+class Tuning(ctypes.Structure):
+    _fields_ = [
+        ("inv_blocks_per_wg", ctypes.c_int),
+        ("hankel_tiles_per_group", ctypes.c_int),
+    ]
+
+
 # Exported C ABI: name -> (restype, argtypes)
 _vp, _i, _ul, _sz = ctypes.c_void_p, ctypes.c_int, ctypes.c_ulong, ctypes.c_size_t
 _f, _ll = ctypes.c_float, ctypes.c_longlong
@@ -84,6 +91,13 @@ SIGNATURES = {
     "msosm_engine_synchronize": (_i, [_vp]),
     "msosm_engine_reset": (_i, [_vp]),
     "msosm_engine_stream": (_i, [_vp, ctypes.POINTER(_vp)]),
+    "msosm_engine_state_size": (_i, [_vp, ctypes.POINTER(_sz)]),
+    "msosm_engine_state_export": (_i, [_vp, _vp, _sz, _i]),
+    "msosm_engine_state_import": (_i, [_vp, _vp, _sz, _i]),
+    "msosm_engine_set_tuning": (_i, [_vp, ctypes.POINTER(Tuning)]),
+    "msosm_engine_get_tuning": (_i, [_vp, ctypes.POINTER(Tuning)]),
+    "msosm_range_push": (_i, [ctypes.c_char_p]),
+    "msosm_range_pop": (_i, []),
     "msosm_engine_set_timing": (_i, [_vp, _i]),
     "msosm_engine_get_timing": (_i, [_vp, _vp, _vp, _i]),
     "msosm_k_uint16_to_complex": (_i, [_vp, _vp, _sz, _vp]),
@@ -309,6 +323,34 @@ class Engine:
 
     def reset(self):
         check(lib().msosm_engine_reset(self.h))
+
+    def state_size(self):
+        n = ctypes.c_size_t()
+        check(lib().msosm_engine_state_size(self.h, ctypes.byref(n)))
+        return int(n.value)
+
+    def state_export(self, out=None):
+        """Stream state image (overlap tail, delay ring, cursor): a numpy
+        uint8 array, or into a given device tensor / numpy buffer."""
+        n = self.state_size()
+        if out is None:
+            out = np.empty(n, np.uint8)
+        nbytes = out.nbytes if isinstance(out, np.ndarray) else out.numel() * out.element_size()
+        check(lib().msosm_engine_state_export(self.h, _ptr(out), nbytes, 1 if _is_device(out) else 0))
+        return out
+
+    def state_import(self, image):
+        nbytes = image.nbytes if isinstance(image, np.ndarray) else image.numel() * image.element_size()
+        check(lib().msosm_engine_state_import(self.h, _ptr(image), nbytes, 1 if _is_device(image) else 0))
+
+    def set_tuning(self, inv_blocks_per_wg=0, hankel_tiles_per_group=0):
+        t = Tuning(inv_blocks_per_wg, hankel_tiles_per_group)
+        check(lib().msosm_engine_set_tuning(self.h, ctypes.byref(t)))
+
+    def get_tuning(self):
+        t = Tuning()
+        check(lib().msosm_engine_get_tuning(self.h, ctypes.byref(t)))
+        return dict(inv_blocks_per_wg=int(t.inv_blocks_per_wg), hankel_tiles_per_group=int(t.hankel_tiles_per_group))
 
     def set_timing(self, on=True):
         check(lib().msosm_engine_set_timing(self.h, 1 if on else 0))

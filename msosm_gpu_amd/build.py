@@ -84,7 +84,8 @@ def build_library(jobs=None):
     if failed:
         raise RuntimeError(f"compile failed: {' '.join(failed)}")
     tmp = LIB + ".tmp"
-    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, *objs])
+    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, *objs, "-L/opt/rocm/lib",
+          "-lrocprofiler-sdk-roctx"])
     os.replace(tmp, LIB)
     return LIB
 

@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -23,6 +24,7 @@
 #include "engine_internal.h"
 #include "kernels.h"
 #include "msosm_c.h"
+#include "trace.h"
 
 namespace {
 
@@ -49,6 +51,10 @@ struct msosm_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     size_t nodes = 0;
+    // host-side engine positions after one replayed pass: capture walks each
+    // engine's cursor through the pass without running it, so the cursors
+    // are put back after capture and moved to these after each launch
+    std::vector<msosm::EngineCursor> after_pass;
 };
 
 namespace {
@@ -136,6 +142,10 @@ int msosm_batch_create(msosm_engine *const *engines, int n, const void *const *i
         for (int j = 0; j < i; j++)
             if (engines[j] == engines[i])
                 return fail(msosm::set_error(MSOSM_ERR_ARG, "batch_create: engine %d listed twice", i));
+        if (msosm::engine_timing(engines[i]))
+            return fail(msosm::set_error(MSOSM_ERR_ARG, "batch_create: engine %d has per-stage timing on (its "
+                                                        "event records cannot be captured); turn it off first",
+                                         i));
         b->engines.push_back(engines[i]);
         b->inputs.push_back((const uint32_t *)inputs[i]);
         b->samples.push_back(num_samples[i]);
@@ -171,6 +181,13 @@ int msosm_batch_create(msosm_engine *const *engines, int n, const void *const *i
     for (auto *e : b->engines)
         if (hipStreamSynchronize(msosm::engine_stream(e)) != hipSuccess)
             return fail(msosm::set_error(MSOSM_ERR_HIP, "batch_create: engine stream not idle"));
+    std::vector<msosm::EngineCursor> before;
+    for (auto *e : b->engines)
+        before.push_back(msosm::engine_cursor(e));
+    auto restore = [&]() {
+        for (size_t i = 0; i < b->engines.size(); i++)
+            msosm::engine_set_cursor(b->engines[i], before[i]);
+    };
     if (hipStreamBeginCapture(b->origin, hipStreamCaptureModeThreadLocal) != hipSuccess)
         return fail(msosm::set_error(MSOSM_ERR_HIP, "batch_create: begin capture"));
     int rc = MSOSM_OK;
@@ -191,6 +208,9 @@ int msosm_batch_create(msosm_engine *const *engines, int n, const void *const *i
     hipGraph_t g = nullptr;
     hipError_t cerr = hipStreamEndCapture(b->origin, &g);
     b->graph = g;
+    for (auto *e : b->engines)
+        b->after_pass.push_back(msosm::engine_cursor(e));
+    restore();
     if (rc != MSOSM_OK)
         return fail(rc);
     if (herr != hipSuccess || cerr != hipSuccess || !g)
@@ -208,7 +228,10 @@ int msosm_batch_launch(msosm_batch *b)
 {
     if (!b || !b->exec)
         return msosm::set_error(MSOSM_ERR_ARG, "batch_launch: bad batch");
+    msosm::TraceRange range("msosm_batch_launch");
     BHIP(hipGraphLaunch(b->exec, b->origin));
+    for (size_t i = 0; i < b->engines.size(); i++)
+        msosm::engine_set_cursor(b->engines[i], b->after_pass[i]);
     return MSOSM_OK;
 }
 
@@ -273,6 +296,7 @@ int msosm_autotune_fftpoint(const msosm_plan_desc *desc, unsigned long min_fftpo
         int form;
     };
     static std::map<std::string, Choice> cache;
+    static std::mutex cache_mu; // held for lookups and inserts, not while timing
     if (!desc || !best_fftpoint || calls < 1 || target_samples < 1)
         return msosm::set_error(MSOSM_ERR_ARG, "autotune: bad argument");
     std::string key;
@@ -286,14 +310,17 @@ int msosm_autotune_fftpoint(const msosm_plan_desc *desc, unsigned long min_fftpo
             key += buf;
         }
     }
-    auto it = cache.find(key);
-    if (it != cache.end()) {
-        *best_fftpoint = it->second.fftpoint;
-        if (best_ns_per_dm_sample)
-            *best_ns_per_dm_sample = it->second.ns;
-        if (best_form)
-            *best_form = it->second.form;
-        return MSOSM_OK;
+    {
+        std::lock_guard<std::mutex> lk(cache_mu);
+        auto it = cache.find(key);
+        if (it != cache.end()) {
+            *best_fftpoint = it->second.fftpoint;
+            if (best_ns_per_dm_sample)
+                *best_ns_per_dm_sample = it->second.ns;
+            if (best_form)
+                *best_form = it->second.form;
+            return MSOSM_OK;
+        }
     }
     hipStream_t s = (hipStream_t)stream;
     unsigned long best = 0;
@@ -384,7 +411,10 @@ int msosm_autotune_fftpoint(const msosm_plan_desc *desc, unsigned long min_fftpo
     if (!best)
         return msosm::set_error(MSOSM_ERR_UNSUPPORTED, "autotune: no FFT size in [%lu, %lu] fits these DM trials",
                                 min_fftpoint, max_fftpoint);
-    cache[key] = {best, best_ns, best_f};
+    {
+        std::lock_guard<std::mutex> lk(cache_mu);
+        cache[key] = {best, best_ns, best_f};
+    }
     *best_fftpoint = best;
     if (best_ns_per_dm_sample)
         *best_ns_per_dm_sample = best_ns;

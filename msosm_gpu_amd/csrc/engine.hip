@@ -27,6 +27,7 @@
 #include "mfma_fft.h"
 #include "msosm_c.h"
 #include "plan.h"
+#include "trace.h"
 
 namespace {
 thread_local std::string g_last_error;
@@ -116,11 +117,13 @@ struct msosm_engine {
     int hankel_mode = -1;
     int kmax = 0;                   // largest delay + 1 over all trials
     int hankel_tpg = 0;             // DM tiles per Hankel workgroup (grid z groups)
+    msosm_tuning tuning{};          // launch-shape requests (0 = the built-in rules)
     int hankel_kspan = 0;           // widest delay band one Hankel workgroup stages
     std::vector<int> hankel_lo_h;   // lowest delay per (trial group, bin group)
     int *hankel_lo = nullptr;       // its device copy
     long long g0 = 0;               // global index of the next block
     bool have_output = false;
+    uint64_t plan_hash = 0;         // identity of the plan (state import check)
     // host input: copies run on copy_stream into alternate halves of in_stage;
     // stage_free[i] is recorded on the engine stream after the kernels that
     // read half i, and the next copy into that half waits for it
@@ -309,6 +312,28 @@ int hankel_mode_of(int form)
            : form == MSOSM_FORM_HANKEL_BF16 ? msosm::HANKEL_BF16 : msosm::HANKEL_BF16X3;
 }
 
+// FNV-1a over what determines an engine's tables and buffer shapes: the
+// trials' (bw, dm, f0), FFT size, blocks per call, method, chirp mode, form.
+uint64_t plan_identity(const msosm::Plan &p)
+{
+    uint64_t h = 1469598103934665603ull;
+    auto mix = [&](const void *d, size_t n) {
+        const unsigned char *c = static_cast<const unsigned char *>(d);
+        for (size_t i = 0; i < n; i++) {
+            h ^= c[i];
+            h *= 1099511628211ull;
+        }
+    };
+    const int ints[5] = {p.num_dms, p.count, p.desc.method, p.desc.chirp, p.desc.form};
+    mix(ints, sizeof ints);
+    mix(&p.N, sizeof p.N);
+    for (int i = 0; i < p.num_dms; i++) {
+        const float v[3] = {p.dms[i].bw, p.dms[i].dm, p.dms[i].f0};
+        mix(v, sizeof v);
+    }
+    return h;
+}
+
 int create_engine(const msosm::Plan &plan, void *stream, msosm_engine **out)
 {
     msosm_engine *e = new msosm_engine();
@@ -322,6 +347,12 @@ int create_engine(const msosm::Plan &plan, void *stream, msosm_engine **out)
         free_engine(e);
         return rc;
     };
+    if (e->form == MSOSM_FORM_HANKEL_BF16)
+        return bail(set_error(MSOSM_ERR_UNSUPPORTED,
+                              "MSOSM_FORM_HANKEL_BF16 (single BF16 operands, 2^-8 relative rounding) misses the 1e-3 "
+                              "mixed-precision gate; use MSOSM_FORM_HANKEL_BF16X3 (hi+lo split) or "
+                              "MSOSM_FORM_HANKEL_F16"));
+    e->plan_hash = plan_identity(plan);
     if (e->form != MSOSM_FORM_PLAIN_F32 && !msosm::fused_supported(e->log2n))
         return bail(set_error(MSOSM_ERR_UNSUPPORTED, "fused forms support FFT sizes 4..16384 (got %lu); use "
                                                      "MSOSM_FORM_PLAIN_F32", N));
@@ -358,7 +389,7 @@ int create_engine(const msosm::Plan &plan, void *stream, msosm_engine **out)
                                   "Hankel forms hold up to %d delay segments per bin in LDS (plan has %d); use a "
                                   "larger FFT size or a fused form",
                                   msosm::hankel_max_kmax(e->hankel_mode), km));
-        msosm::hankel_groups(e->hankel_mode, p.delay.data(), (int)p.num_dms, (int)N, km, &e->hankel_tpg,
+        msosm::hankel_groups(e->hankel_mode, p.delay.data(), (int)p.num_dms, (int)N, km, 0, &e->hankel_tpg,
                              &e->hankel_kspan, &e->hankel_lo_h);
     }
     int ndev = 0;
@@ -559,7 +590,8 @@ int filter_hankel(msosm_engine *e, const uint32_t *cur)
         // FP16 products are (X/N)(H/(2M)), FP32 products X H: both give
         // IDFT(X H) / N^2 after these scales
         const float scale = f16 ? 1.0f : 1.0f / ((float)N * (float)N);
-        HIP_TRY(msosm::launch_inv_dense(e->ybuf, e->out, f16, e->log2n, (long long)ndm * count, scale, s));
+        HIP_TRY(msosm::launch_inv_dense(e->ybuf, e->out, f16, e->log2n, (long long)ndm * count, scale,
+                                        e->tuning.inv_blocks_per_wg, s));
     }
     e->g0 += count;
     return MSOSM_OK;
@@ -582,10 +614,11 @@ int filter_fused(msosm_engine *e, const uint32_t *cur)
         StageScope st(e, MSOSM_STAGE_FFT_INV);
         if (mfma)
             HIP_TRY(msosm::launch_inv_mfma(static_cast<const __half2 *>(e->xring), e->Hp, e->dpk_mfma, e->mfma_tab,
-                                           e->out, count, ndm, e->g0, e->rx, s));
+                                           e->out, count, ndm, e->g0, e->rx, e->tuning.inv_blocks_per_wg, s));
         else
             HIP_TRY(msosm::launch_inv_fused(e->xring, f16 ? (const void *)e->H16 : (const void *)e->H32, e->delay16,
-                                            e->delay_pk, e->out, f16, e->log2n, count, ndm, e->g0, e->rx, s));
+                                            e->delay_pk, e->out, f16, e->log2n, count, ndm, e->g0, e->rx,
+                                            e->tuning.inv_blocks_per_wg, s));
     }
     e->g0 += count;
     return MSOSM_OK;
@@ -613,6 +646,56 @@ int msosm::engine_reset_async(msosm_engine *e)
 hipStream_t msosm::engine_stream(msosm_engine *e) { return e->stream; }
 float2 *msosm::engine_output(msosm_engine *e) { return e->out; }
 const msosm::Plan &msosm::engine_plan(msosm_engine *e) { return e->plan; }
+msosm::EngineCursor msosm::engine_cursor(const msosm_engine *e)
+{
+    msosm::EngineCursor c;
+    c.g0 = e->g0;
+    c.current_index = e->current_index;
+    c.have_output = e->have_output;
+    return c;
+}
+void msosm::engine_set_cursor(msosm_engine *e, const msosm::EngineCursor &c)
+{
+    e->g0 = c.g0;
+    e->current_index = c.current_index;
+    e->have_output = c.have_output;
+}
+bool msosm::engine_timing(const msosm_engine *e) { return e->timing; }
+
+namespace {
+// Stream state of an engine as one byte image: this header, then the
+// sections of state_sections() in order.
+struct StateHeader {
+    char magic[8];          // "MSOSMST1"
+    uint64_t plan_hash;
+    int32_t form, num_dms, count, ring_slots;
+    uint64_t M;
+    int64_t g0;
+    int32_t current_index, have_output;
+    uint64_t section_bytes[2];
+};
+constexpr char kStateMagic[8] = {'M', 'S', 'O', 'S', 'M', 'S', 'T', '1'};
+
+// The history a filter call reads: reference-structured form -- the last M
+// input samples (head of the next call's overlapped input) and the product
+// ring; shared-ring forms -- both halves of the packed-input tail and the
+// spectra ring.
+void state_sections(msosm_engine *e, void *ptr[2], size_t bytes[2])
+{
+    const msosm::Plan &p = e->plan;
+    if (e->form == MSOSM_FORM_PLAIN_F32) {
+        ptr[0] = e->inbuf + (size_t)p.count * p.M;
+        bytes[0] = sizeof(float2) * p.M;
+        ptr[1] = e->ring;
+        bytes[1] = sizeof(float2) * (size_t)p.num_dms * p.R * p.N;
+    } else {
+        ptr[0] = e->tail;
+        bytes[0] = sizeof(uint32_t) * 2 * p.M;
+        ptr[1] = e->xring;
+        bytes[1] = (form_f16_ring(e->form) ? sizeof(__half2) : sizeof(float2)) * (size_t)e->rx * p.N;
+    }
+}
+} // namespace
 
 // ===========================================================================
 // C ABI
@@ -728,6 +811,7 @@ int msosm_engine_filter_u16(msosm_engine *e, const void *input, int where)
 {
     if (!e || !input)
         return set_error(MSOSM_ERR_ARG, "filter_u16: NULL argument");
+    msosm::TraceRange range("msosm_engine_filter_u16");
     const msosm::Plan &p = e->plan;
     const uint32_t *cur = (const uint32_t *)input;
     const int slot = e->stage_slot;
@@ -775,6 +859,7 @@ int msosm_engine_output_c64(msosm_engine *e, void *dst, int where)
 {
     if (!e || !dst)
         return set_error(MSOSM_ERR_ARG, "output_c64: NULL argument");
+    msosm::TraceRange range("msosm_engine_output_c64");
     const msosm::Plan &p = e->plan;
     size_t bytes = sizeof(float2) * (size_t)p.num_dms * p.count * p.M;
     HIP_TRY(hipMemcpyAsync(dst, e->out, bytes, where == 1 ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
@@ -788,6 +873,7 @@ int msosm_engine_output_u16(msosm_engine *e, void *dst, int where)
 {
     if (!e || !dst)
         return set_error(MSOSM_ERR_ARG, "output_u16: NULL argument");
+    msosm::TraceRange range("msosm_engine_output_u16");
     const msosm::Plan &p = e->plan;
     size_t n = (size_t)p.num_dms * p.count * p.M;
     HIP_TRY(msosm::launch_pack(e->out_u16, e->out, n, e->stream));
@@ -832,6 +918,140 @@ int msosm_engine_reset(msosm_engine *e)
     if (rc != MSOSM_OK)
         return rc;
     HIP_TRY(hipStreamSynchronize(e->stream));
+    return MSOSM_OK;
+}
+
+int msosm_engine_state_size(msosm_engine *e, size_t *bytes)
+{
+    if (!e || !bytes)
+        return set_error(MSOSM_ERR_ARG, "state_size: NULL argument");
+    void *ptr[2];
+    size_t sec[2];
+    state_sections(e, ptr, sec);
+    *bytes = sizeof(StateHeader) + sec[0] + sec[1];
+    return MSOSM_OK;
+}
+
+int msosm_engine_state_export(msosm_engine *e, void *dst, size_t bytes, int where)
+{
+    if (!e || !dst || (where != 0 && where != 1))
+        return set_error(MSOSM_ERR_ARG, "state_export: bad argument");
+    msosm::TraceRange range("msosm_engine_state_export");
+    void *ptr[2];
+    size_t sec[2];
+    state_sections(e, ptr, sec);
+    if (bytes < sizeof(StateHeader) + sec[0] + sec[1])
+        return set_error(MSOSM_ERR_ARG, "state_export: buffer of %zu bytes, need %zu", bytes,
+                         sizeof(StateHeader) + sec[0] + sec[1]);
+    StateHeader h{};
+    std::memcpy(h.magic, kStateMagic, sizeof h.magic);
+    h.plan_hash = e->plan_hash;
+    h.form = e->form;
+    h.num_dms = e->plan.num_dms;
+    h.count = e->plan.count;
+    h.ring_slots = e->form == MSOSM_FORM_PLAIN_F32 ? e->plan.R : e->rx;
+    h.M = e->plan.M;
+    h.g0 = e->g0;
+    h.current_index = e->current_index;
+    h.have_output = e->have_output ? 1 : 0;
+    h.section_bytes[0] = sec[0];
+    h.section_bytes[1] = sec[1];
+    char *d = static_cast<char *>(dst);
+    const hipMemcpyKind kind = where == 1 ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    if (where == 1)
+        HIP_TRY(hipMemcpyAsync(d, &h, sizeof h, hipMemcpyHostToDevice, e->stream));
+    else
+        std::memcpy(d, &h, sizeof h);
+    HIP_TRY(hipMemcpyAsync(d + sizeof h, ptr[0], sec[0], kind, e->stream));
+    HIP_TRY(hipMemcpyAsync(d + sizeof h + sec[0], ptr[1], sec[1], kind, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    return MSOSM_OK;
+}
+
+int msosm_engine_state_import(msosm_engine *e, const void *src, size_t bytes, int where)
+{
+    if (!e || !src || (where != 0 && where != 1))
+        return set_error(MSOSM_ERR_ARG, "state_import: bad argument");
+    msosm::TraceRange range("msosm_engine_state_import");
+    void *ptr[2];
+    size_t sec[2];
+    state_sections(e, ptr, sec);
+    if (bytes < sizeof(StateHeader) + sec[0] + sec[1])
+        return set_error(MSOSM_ERR_ARG, "state_import: image of %zu bytes, this engine's state is %zu", bytes,
+                         sizeof(StateHeader) + sec[0] + sec[1]);
+    const char *s = static_cast<const char *>(src);
+    StateHeader h{};
+    if (where == 1)
+        HIP_TRY(hipMemcpy(&h, s, sizeof h, hipMemcpyDeviceToHost));
+    else
+        std::memcpy(&h, s, sizeof h);
+    if (std::memcmp(h.magic, kStateMagic, sizeof h.magic) != 0)
+        return set_error(MSOSM_ERR_ARG, "state_import: not an engine state image");
+    const int slots = e->form == MSOSM_FORM_PLAIN_F32 ? e->plan.R : e->rx;
+    if (h.plan_hash != e->plan_hash || h.form != e->form || h.num_dms != e->plan.num_dms ||
+        h.count != e->plan.count || h.ring_slots != slots || h.M != e->plan.M || h.section_bytes[0] != sec[0] ||
+        h.section_bytes[1] != sec[1])
+        return set_error(MSOSM_ERR_ARG, "state_import: the image belongs to a different plan (trials, FFT size, "
+                                        "blocks per call, method, chirp mode or form differ)");
+    const hipMemcpyKind kind = where == 1 ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    HIP_TRY(hipMemcpyAsync(ptr[0], s + sizeof h, sec[0], kind, e->stream));
+    HIP_TRY(hipMemcpyAsync(ptr[1], s + sizeof h + sec[0], sec[1], kind, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    e->g0 = h.g0;
+    e->current_index = h.current_index;
+    e->have_output = h.have_output != 0;
+    return MSOSM_OK;
+}
+
+int msosm_engine_set_tuning(msosm_engine *e, const msosm_tuning *t)
+{
+    if (!e || !t || t->inv_blocks_per_wg < 0 || t->inv_blocks_per_wg > 64 || t->hankel_tiles_per_group < 0)
+        return set_error(MSOSM_ERR_ARG, "set_tuning: bad argument (0 <= inv_blocks_per_wg <= 64, "
+                                        "hankel_tiles_per_group >= 0)");
+    if (form_hankel(e->form) && t->hankel_tiles_per_group != e->tuning.hankel_tiles_per_group) {
+        int tpg = 0, kspan = 0;
+        std::vector<int> lo;
+        msosm::hankel_groups(e->hankel_mode, e->plan.delay.data(), e->plan.num_dms, (int)e->plan.N, e->kmax,
+                             t->hankel_tiles_per_group, &tpg, &kspan, &lo);
+        int *lo_d = nullptr;
+        HIP_TRY(dev_alloc(&lo_d, lo.size()));
+        hipError_t herr = hipMemcpy(lo_d, lo.data(), sizeof(int) * lo.size(), hipMemcpyHostToDevice);
+        if (herr != hipSuccess) {
+            (void)hipFree(lo_d);
+            return set_error(MSOSM_ERR_HIP, "set_tuning: band table upload: %s", hipGetErrorString(herr));
+        }
+        HIP_TRY(hipStreamSynchronize(e->stream)); // the old table may still be read
+        (void)hipFree(e->hankel_lo);
+        e->hankel_lo = lo_d;
+        e->hankel_lo_h.swap(lo);
+        e->hankel_tpg = tpg;
+        e->hankel_kspan = kspan;
+    }
+    e->tuning = *t;
+    return MSOSM_OK;
+}
+
+int msosm_engine_get_tuning(msosm_engine *e, msosm_tuning *t)
+{
+    if (!e || !t)
+        return set_error(MSOSM_ERR_ARG, "get_tuning: NULL argument");
+    *t = e->tuning;
+    if (form_hankel(e->form))
+        t->hankel_tiles_per_group = e->hankel_tpg;
+    return MSOSM_OK;
+}
+
+int msosm_range_push(const char *name)
+{
+    roctxRangePushA(name ? name : "");
+    return MSOSM_OK;
+}
+
+int msosm_range_pop(void)
+{
+    roctxRangePop();
     return MSOSM_OK;
 }
 
@@ -937,7 +1157,7 @@ int msosm_k_inverse_discard(const void *Y, float *out, int y_f16, int log2n, siz
 {
     if (!Y || !out || log2n < 2 || log2n > 14 || units < 1)
         return set_error(MSOSM_ERR_ARG, "inverse_discard: bad argument (2 <= log2n <= 14, units >= 1)");
-    HIP_TRY(msosm::launch_inv_dense(Y, (float2 *)out, y_f16 != 0, log2n, (long long)units, scale,
+    HIP_TRY(msosm::launch_inv_dense(Y, (float2 *)out, y_f16 != 0, log2n, (long long)units, scale, 0,
                                     (hipStream_t)stream));
     return MSOSM_OK;
 }

@@ -21,6 +21,7 @@
 #include "common.h"
 #include "kernels.h"
 #include "msosm_c.h"
+#include "trace.h"
 
 namespace {
 
@@ -471,6 +472,7 @@ int msosm_fold_intensity(msosm_fold *f, const void *pol1, const void *pol2)
 {
     if (!f || !pol1)
         return ferr(MSOSM_ERR_ARG, "fold_intensity: NULL argument");
+    msosm::TraceRange range("msosm_fold_intensity");
     FHIP(msosm::launch_intensity((const float2 *)pol1, (const float2 *)pol2, f->intensity, f->size, f->stream));
     return MSOSM_OK;
 }
@@ -499,6 +501,7 @@ int msosm_fold_fold_data(msosm_fold *f)
 {
     if (!f)
         return ferr(MSOSM_ERR_ARG, "fold_fold_data: NULL fold");
+    msosm::TraceRange range("msosm_fold_fold_data");
     const unsigned long P = f->period_samples;
     const unsigned long NEG1 = (unsigned long)-1;
     if (f->discard_count == 0 && !f->ready)
@@ -672,16 +675,12 @@ int msosm_fold_phase(const void *a, const void *b, size_t n, int ndm, size_t str
                                    "stride >= n)");
     if (ndm > 65535)
         return ferr(MSOSM_ERR_ARG, "fold_phase: at most 65535 DM trials per launch");
+    msosm::TraceRange range("msosm_fold_phase");
     // two samples per lane and load when every row starts 16-byte aligned
     const bool wide = ((uintptr_t)a % 16 == 0) && (!b || (uintptr_t)b % 16 == 0) && (ndm == 1 || stride % 2 == 0);
     // contiguous chunks of at least 8192 samples (a multiple of one step of
-    // the 4 waves), at most 4096 chunks per trial; MSOSM_FOLD_CHUNK overrides
-    // the minimum (tuning runs)
-    static const long long min_chunk = [] {
-        const char *e = getenv("MSOSM_FOLD_CHUNK");
-        long long c = e ? atoll(e) : 8192;
-        return c >= 1024 ? c : 8192;
-    }();
+    // the 4 waves), at most 4096 chunks per trial
+    constexpr long long min_chunk = 8192;
     const long long step = (long long)(kFoldThreads / 64) * 64 * (wide ? 2 : 1) * kFoldUnroll;
     long long chunk = min_chunk;
     while ((long long)(n + chunk - 1) / chunk > 4096)

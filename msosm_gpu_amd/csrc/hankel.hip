@@ -476,12 +476,13 @@ hipError_t hankel_launch(const void *ring, const void *H, const uint16_t *delay,
 }
 
 template <int LOG2N>
-hipError_t inv_dense(const void *Y, float2 *out, bool f16y, long long units, float scale, hipStream_t s)
+hipError_t inv_dense(const void *Y, float2 *out, bool f16y, long long units, float scale, int nbpw_req, hipStream_t s)
 {
     constexpr int E = fft_e(LOG2N);
     constexpr int threads = FftShape<LOG2N, E>::NT * fft_tpb(LOG2N);
     if constexpr (fft_tpb(LOG2N) == 1) {
-        const int nbpw = blocks_per_wg((int)std::min<long long>(units, INT_MAX), 1, 32);
+        const int nbpw = nbpw_req > 0 ? (int)std::min<long long>(nbpw_req, units)
+                                      : blocks_per_wg((int)std::min<long long>(units, INT_MAX), 1, 32);
         const long long grid = (units + nbpw - 1) / nbpw;
         if (f16y)
             hipLaunchKernelGGL((k_inv_dense_nb<LOG2N, true>), dim3((unsigned)grid), dim3(threads), 0, s, Y, out,
@@ -535,7 +536,7 @@ static size_t hankel_lds_of(int mode, int U)
                                  : hankel_lds_bytes<HANKEL_BF16X3>(U);
 }
 
-void hankel_groups(int mode, const int32_t *delay, int ndm, int N, int kmax, int *tpg, int *kspan,
+void hankel_groups(int mode, const int32_t *delay, int ndm, int N, int kmax, int tpg_req, int *tpg, int *kspan,
                    std::vector<int> *band_lo)
 {
     const int nbin = hankel_bins_per_wg(mode), ngrp = N / nbin;
@@ -565,12 +566,17 @@ void hankel_groups(int mode, const int32_t *delay, int ndm, int N, int kmax, int
     };
     // the largest group (fewest repeated ring columns) whose LDS leaves room
     // for three workgroups per CU (the staged output alone takes 34 KB for
-    // FP16); MSOSM_HANKEL_TPG overrides
+    // FP16), unless the caller asks for a group size (tuning); a requested
+    // size whose band does not fit the LDS limit is reduced like the rule
     int t = ntiles, w = kmax;
-    const char *env = getenv("MSOSM_HANKEL_TPG");
-    if (env && atoi(env) > 0) {
-        t = std::min(atoi(env), ntiles);
-        w = span(t);
+    if (tpg_req > 0) {
+        t = std::min(tpg_req, ntiles);
+        for (;;) {
+            w = span(t);
+            if (hankel_lds_of(mode, kGTile - 1 + w + 16) <= kHankelMaxLds || t == 1)
+                break;
+            t = (t + 1) / 2;
+        }
     } else {
         for (;;) {
             w = span(t);
@@ -606,9 +612,9 @@ hipError_t launch_hankel(const void *ring, const void *H, const uint16_t *delay,
 }
 
 hipError_t launch_inv_dense(const void *Y, float2 *out, bool f16y, int log2n, long long units, float scale,
-                            hipStream_t s)
+                            int nbpw, hipStream_t s)
 {
-#define CALL_D(L) inv_dense<L>(Y, out, f16y, units, scale, s)
+#define CALL_D(L) inv_dense<L>(Y, out, f16y, units, scale, nbpw, s)
     switch (log2n) {
     case 2: return CALL_D(2);
     case 3: return CALL_D(3);

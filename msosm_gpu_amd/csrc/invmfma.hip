@@ -36,6 +36,8 @@
 #include <hip/hip_runtime.h>
 #include <hip/hip_fp16.h>
 
+#include <algorithm>
+
 #include "fft_device.h"
 #include "kernels.h"
 #include "mfma_fft.h"
@@ -318,9 +320,9 @@ hipError_t launch_inv_mfma_filter(__half2 *dst, const __half2 *src, long long ro
 }
 
 hipError_t launch_inv_mfma(const __half2 *ring, const __half2 *Hp, const uint32_t *dpk, const void *tables,
-                           float2 *out, int count, int ndm, long long g0, int rx, hipStream_t s)
+                           float2 *out, int count, int ndm, long long g0, int rx, int nbpw_req, hipStream_t s)
 {
-    const int nbpw = blocks_per_wg(count, ndm, kNBWide);
+    const int nbpw = nbpw_req > 0 ? std::min(nbpw_req, count) : blocks_per_wg(count, ndm, kNBWide);
     const long long chunks = (count + nbpw - 1) / nbpw;
     const long long grid = chunks * ndm;
     hipLaunchKernelGGL(k_inv_mfma, dim3((unsigned)grid), dim3(kThreads), 0, s, ring, Hp, dpk,

@@ -33,8 +33,10 @@ hipError_t launch_fwd_u16(const uint32_t *cur, const uint32_t *tail, uint32_t *t
 // of one trial per workgroup: the largest power of two up to max_nbpw that
 // leaves at least 512 workgroups for rows x count blocks, else 1.
 int blocks_per_wg(int count, long long rows, int max_nbpw);
+// nbpw: blocks per workgroup for sizes with one transform per workgroup
+// (0 = blocks_per_wg(count, ndm, 32)).
 hipError_t launch_inv_fused(const void *ring, const void *H, const uint16_t *delay, const uint32_t *dpk, float2 *out,
-                            bool f16, int log2n, int count, int ndm, long long g0, int rx, hipStream_t s);
+                            bool f16, int log2n, int count, int ndm, long long g0, int rx, int nbpw, hipStream_t s);
 int fused_delay_threads(int log2n);
 bool fused_uses_packed_delay(int log2n);
 bool fused_supported(int log2n);
@@ -48,7 +50,7 @@ bool fused_supported(int log2n);
 size_t inv_mfma_tables_bytes();
 hipError_t launch_inv_mfma_tables(void *tables, hipStream_t s);
 hipError_t launch_inv_mfma(const __half2 *ring, const __half2 *Hp, const uint32_t *dpk, const void *tables,
-                           float2 *out, int count, int ndm, long long g0, int rx, hipStream_t s);
+                           float2 *out, int count, int ndm, long long g0, int rx, int nbpw, hipStream_t s);
 // Filter rows for launch_inv_mfma from the FP16 rows H/(2M): dst[r][ring_pos(n)]
 // = src[r][n] / 64 (FP16 headroom; the kernel's last DFT carries the 64)
 hipError_t launch_inv_mfma_filter(__half2 *dst, const __half2 *src, long long rows, hipStream_t s);
@@ -69,14 +71,16 @@ hipError_t launch_hankel(const void *ring, const void *H, const uint16_t *delay,
 // tiles of 16 trials per workgroup (grid z), the widest delay band kspan any
 // workgroup stages, and band_lo[group][bin group] its lowest delay (a device
 // copy goes to launch_hankel).  band_lo null stages the whole delay line.
-void hankel_groups(int mode, const int32_t *delay, int ndm, int N, int kmax, int *tpg, int *kspan,
+// tpg_req > 0 asks for that many tiles per group (tuning), 0 = the rule.
+void hankel_groups(int mode, const int32_t *delay, int ndm, int N, int kmax, int tpg_req, int *tpg, int *kspan,
                    std::vector<int> *band_lo);
 int hankel_max_kmax(int mode);
 int hankel_bins_per_wg(int mode);
 // Inverse FFT of dense rows Y[unit][2^log2n] (FP16 or FP32 complex), keep the
 // last half, times scale: out[unit][2^(log2n-1)].
+// nbpw: consecutive rows per workgroup at 2^12..2^14 points (0 = the rule).
 hipError_t launch_inv_dense(const void *Y, float2 *out, bool f16y, int log2n, long long units, float scale,
-                            hipStream_t s);
+                            int nbpw, hipStream_t s);
 
 // Element kernels (reference gpu_kernel.cu semantics).
 hipError_t launch_unpack(float2 *dst, const uint32_t *src, size_t n, hipStream_t s);

@@ -13,6 +13,7 @@
 
 #include <hip/hip_fp16.h>
 
+#include <algorithm>
 #include <cstdlib>
 
 #include "fft_device.h"
@@ -527,15 +528,13 @@ __global__ __launch_bounds__(fft_threads(LOG2N)) void k_inv_fused(
 
 // Fused inverse for one transform per workgroup (2^12..2^14 points): the
 // workgroup runs nbpw consecutive blocks of one DM trial (H and the delays
-// are re-read per block, L2 hits); PREFETCH (a tuning variant) issues the
-// gather of block i+1 before the transform of block i and holds those ring
-// values in registers across it.  The delays
+// are re-read per block, L2 hits).  The delays
 // come in a thread-major packed layout dpk[dm][j][t] = d(t + 2j NT) |
 // d(t + (2j+1) NT) << 16 so each thread reads E/2 coalesced words.  Work
 // items are ordered so that each XCD (workgroups are dealt to the 8 XCDs
 // round robin) gets a contiguous range of DM trials and keeps their tables in
 // its L2.
-template <int LOG2N, int E, bool F16, bool PREFETCH>
+template <int LOG2N, int E, bool F16>
 __global__ __launch_bounds__((1 << LOG2N) / E) void k_inv_fused_nb(
     const void *__restrict__ ring, const void *__restrict__ H, const uint32_t *__restrict__ dpk,
     float2 *__restrict__ out, int count, int ndm, long long g0, int rx, int nbpw)
@@ -591,24 +590,14 @@ __global__ __launch_bounds__((1 << LOG2N) / E) void k_inv_fused_nb(
             }
         }
     };
-    if constexpr (PREFETCH)
-        gather();
     const float scale = 1.0f / ((float)N * (float)N);
     for (int i = 0; i < nb; i++) {
-        if constexpr (!PREFETCH)
-            gather();
+        gather();
         float2 v[E];
 #pragma unroll
         for (int m = 0; m < E; m++)
             v[m] = R::mul(raw[m], hrow[t + m * NT]);
-        if constexpr (PREFETCH) {
-            if (i + 1 < nb) {
-                slot = slot + 1 == rx ? 0 : slot + 1;
-                gather();
-            }
-        } else {
-            slot = slot + 1 == rx ? 0 : slot + 1;
-        }
+        slot = slot + 1 == rx ? 0 : slot + 1;
         fft_block<LOG2N, E, true, ROWS>(v, smem, twlo, twhi, t, twrows);
         float2 *dst = out + ((long long)dm * count + b0 + i) * M;
 #pragma unroll
@@ -802,67 +791,29 @@ int blocks_per_wg(int count, long long rows, int max_nbpw)
 }
 static int inv_blocks_per_wg(int count, int ndm) { return blocks_per_wg(count, ndm, 32); }
 
-// Inverse-kernel variant for the sizes with one transform per workgroup:
-// 0 = one (DM, block) per workgroup, 1 = inv_blocks_per_wg blocks per
-// workgroup (default), 2 = as 1 with the next block's gather prefetched into
-// registers, 3 = as 2 with 16 points per thread for 2^14 (1024 threads),
-// 4 = as 1 with 16 points per thread for 2^14.
-// Measured on MI355X for 256 trials x 64 blocks of 2^14 points (FP16 ring):
-// 2.13 / 0.66 / 1.75 / 1.75 ms per launch for 0-3 -- the prefetch costs
-// register spills that outweigh the overlap.  MSOSM_INV_KERNEL overrides
-// the choice.
-static int inv_variant()
-{
-    static int v = [] {
-        const char *e = getenv("MSOSM_INV_KERNEL");
-        int x = e ? atoi(e) : 1;
-        return (x >= 0 && x <= 4) ? x : 1;
-    }();
-    return v;
-}
-
-template <int LOG2N, int E, bool F16, bool PF>
+template <int LOG2N, int E, bool F16>
 static void launch_nb(const void *ring, const void *H, const uint32_t *dpk, float2 *out, int count, int ndm,
-                      long long g0, int rx, hipStream_t s)
+                      long long g0, int rx, int nbpw_req, hipStream_t s)
 {
     constexpr int threads = (1 << LOG2N) / E;
-    const int nbpw = inv_blocks_per_wg(count, ndm);
+    const int nbpw = nbpw_req > 0 ? std::min(nbpw_req, count) : inv_blocks_per_wg(count, ndm);
     const long long chunks = (count + nbpw - 1) / nbpw;
     const long long grid = chunks * ndm;
-    hipLaunchKernelGGL((k_inv_fused_nb<LOG2N, E, F16, PF>), dim3((unsigned)grid), dim3(threads), 0, s, ring, H, dpk,
-                       out, count, ndm, g0, rx, nbpw);
+    hipLaunchKernelGGL((k_inv_fused_nb<LOG2N, E, F16>), dim3((unsigned)grid), dim3(threads), 0, s, ring, H, dpk, out,
+                       count, ndm, g0, rx, nbpw);
 }
 
 template <int LOG2N>
 static hipError_t inv_fused(const void *ring, const void *H, const uint16_t *delay, const uint32_t *dpk, float2 *out,
-                            bool f16, int count, int ndm, long long g0, int rx, hipStream_t s)
+                            bool f16, int count, int ndm, long long g0, int rx, int nbpw, hipStream_t s)
 {
     constexpr int E = fft_e(LOG2N);
     constexpr int threads = FftShape<LOG2N, E>::NT * fft_tpb(LOG2N);
-    const int var = inv_variant();
-    if (fft_tpb(LOG2N) == 1 && var > 0) {
-        if constexpr (fft_tpb(LOG2N) == 1) {
-            if (LOG2N == 14 && var >= 3) {
-                if (f16 && var == 3)
-                    launch_nb<LOG2N, 16, true, true>(ring, H, dpk, out, count, ndm, g0, rx, s);
-                else if (f16)
-                    launch_nb<LOG2N, 16, true, false>(ring, H, dpk, out, count, ndm, g0, rx, s);
-                else if (var == 3)
-                    launch_nb<LOG2N, 16, false, true>(ring, H, dpk, out, count, ndm, g0, rx, s);
-                else
-                    launch_nb<LOG2N, 16, false, false>(ring, H, dpk, out, count, ndm, g0, rx, s);
-            } else if (f16) {
-                if (var >= 2)
-                    launch_nb<LOG2N, E, true, true>(ring, H, dpk, out, count, ndm, g0, rx, s);
-                else
-                    launch_nb<LOG2N, E, true, false>(ring, H, dpk, out, count, ndm, g0, rx, s);
-            } else {
-                if (var >= 2)
-                    launch_nb<LOG2N, E, false, true>(ring, H, dpk, out, count, ndm, g0, rx, s);
-                else
-                    launch_nb<LOG2N, E, false, false>(ring, H, dpk, out, count, ndm, g0, rx, s);
-            }
-        }
+    if constexpr (fft_tpb(LOG2N) == 1) {
+        if (f16)
+            launch_nb<LOG2N, E, true>(ring, H, dpk, out, count, ndm, g0, rx, nbpw, s);
+        else
+            launch_nb<LOG2N, E, false>(ring, H, dpk, out, count, ndm, g0, rx, nbpw, s);
     } else {
         const long long units = (long long)count * ndm;
         const long long grid = (units + fft_tpb(LOG2N) - 1) / fft_tpb(LOG2N);
@@ -903,18 +854,13 @@ hipError_t launch_fwd_u16(const uint32_t *cur, const uint32_t *tail, uint32_t *t
 #undef CALL_F
 }
 
-int fused_delay_threads(int log2n)
-{
-    if (log2n == 14 && inv_variant() >= 3)
-        return (1 << log2n) / 16;
-    return (1 << log2n) / fft_e(log2n);
-}
+int fused_delay_threads(int log2n) { return (1 << log2n) / fft_e(log2n); }
 bool fused_uses_packed_delay(int log2n) { return log2n >= 2 && log2n <= kMaxLog2SinglePass && fft_tpb(log2n) == 1; }
 
 hipError_t launch_inv_fused(const void *ring, const void *H, const uint16_t *delay, const uint32_t *dpk, float2 *out,
-                            bool f16, int log2n, int count, int ndm, long long g0, int rx, hipStream_t s)
+                            bool f16, int log2n, int count, int ndm, long long g0, int rx, int nbpw, hipStream_t s)
 {
-#define CALL_I(L) inv_fused<L>(ring, H, delay, dpk, out, f16, count, ndm, g0, rx, s)
+#define CALL_I(L) inv_fused<L>(ring, H, delay, dpk, out, f16, count, ndm, g0, rx, nbpw, s)
     MSOSM_DISPATCH_2_14(log2n, CALL_I)
 #undef CALL_I
 }

@@ -31,6 +31,7 @@
 #include "engine_internal.h"
 #include "kernels.h"
 #include "msosm_c.h"
+#include "trace.h"
 
 namespace {
 
@@ -42,13 +43,6 @@ struct Slot {
     hipEvent_t staged = nullptr; // staging copy done (engine stream)
     hipEvent_t ready = nullptr;  // host copy done (copy stream)
 };
-
-#define RHIP(expr)                                                                                               \
-    do {                                                                                                         \
-        hipError_t e__ = (expr);                                                                                 \
-        if (e__ != hipSuccess)                                                                                   \
-            return msosm::set_error(MSOSM_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e__));              \
-    } while (0)
 
 } // namespace
 
@@ -142,6 +136,7 @@ int msosm_outring_push(msosm_outring *r, msosm_engine *e, int format, int timeou
 {
     if (!r || !e || (format != 0 && format != 1))
         return msosm::set_error(MSOSM_ERR_ARG, "outring_push: bad argument (format 0 complex64, 1 uint16)");
+    msosm::TraceRange range("msosm_outring_push");
     const msosm::Plan &p = msosm::engine_plan(e);
     const size_t n = (size_t)p.num_dms * p.count * p.M;
     const size_t bytes = n * (format == 0 ? sizeof(float2) : sizeof(uint32_t));
@@ -159,15 +154,37 @@ int msosm_outring_push(msosm_outring *r, msosm_engine *e, int format, int timeou
     Slot &s = r->slots[idx];
     hipStream_t es = msosm::engine_stream(e);
     const float2 *out = msosm::engine_output(e);
-    if (format == 0) {
-        RHIP(hipMemcpyAsync(s.dev, out, bytes, hipMemcpyDeviceToDevice, es));
-    } else {
-        RHIP(msosm::launch_pack(static_cast<uint32_t *>(s.dev), out, n, es));
+    // a failure after the slot was taken hands it back (at the front, so the
+    // next push reuses it) instead of shrinking the ring for good
+    hipError_t herr = hipSuccess;
+    const char *what = nullptr;
+    auto step = [&](hipError_t x, const char *name) {
+        if (herr == hipSuccess && x != hipSuccess) {
+            herr = x;
+            what = name;
+        }
+        return herr == hipSuccess;
+    };
+    if (format == 0)
+        step(hipMemcpyAsync(s.dev, out, bytes, hipMemcpyDeviceToDevice, es), "staging copy");
+    else
+        step(msosm::launch_pack(static_cast<uint32_t *>(s.dev), out, n, es), "uint16 pack");
+    if (herr == hipSuccess)
+        step(hipEventRecord(s.staged, es), "staged event");
+    if (herr == hipSuccess)
+        step(hipStreamWaitEvent(r->copy_stream, s.staged, 0), "copy-stream wait");
+    if (herr == hipSuccess)
+        step(hipMemcpyAsync(s.host, s.dev, bytes, hipMemcpyDeviceToHost, r->copy_stream), "host copy");
+    if (herr == hipSuccess)
+        step(hipEventRecord(s.ready, r->copy_stream), "ready event");
+    if (herr != hipSuccess) {
+        {
+            std::lock_guard<std::mutex> lk(r->mu);
+            r->free_slots.push_front(idx);
+        }
+        r->cv.notify_all();
+        return msosm::set_error(MSOSM_ERR_HIP, "outring_push: %s failed: %s", what, hipGetErrorString(herr));
     }
-    RHIP(hipEventRecord(s.staged, es));
-    RHIP(hipStreamWaitEvent(r->copy_stream, s.staged, 0));
-    RHIP(hipMemcpyAsync(s.host, s.dev, bytes, hipMemcpyDeviceToHost, r->copy_stream));
-    RHIP(hipEventRecord(s.ready, r->copy_stream));
     {
         std::lock_guard<std::mutex> lk(r->mu);
         s.bytes = bytes;
@@ -196,7 +213,17 @@ int msosm_outring_pop(msosm_outring *r, int timeout_ms, const void **data, size_
         r->held = idx;
     }
     Slot &s = r->slots[idx];
-    RHIP(hipEventSynchronize(s.ready));
+    hipError_t herr = hipEventSynchronize(s.ready);
+    if (herr != hipSuccess) {
+        // the slot's content is not valid: free it rather than hand it out
+        {
+            std::lock_guard<std::mutex> lk(r->mu);
+            r->held = -1;
+            r->free_slots.push_back(idx);
+        }
+        r->cv.notify_all();
+        return msosm::set_error(MSOSM_ERR_HIP, "outring_pop: waiting for slot %d: %s", idx, hipGetErrorString(herr));
+    }
     *data = s.host;
     if (bytes)
         *bytes = s.bytes;

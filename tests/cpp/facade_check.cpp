@@ -70,7 +70,9 @@ int main(int argc, char **argv)
     for (unsigned long c = 0; c + per <= L; c += per) {
         conc.filter_block_uint16(in.data() + c);
         cbuf[tot] = Complex{1.0f, 1.0f};
+        PUSH_RANGE("Output Copy", 1) // as the reference drivers mark it
         conc.get_output(cbuf.data());
+        POP_RANGE
         conc.wait_for_cpu = false; // what the consumer thread does once it has the block
         if (cbuf[tot].x != 0.0f)
             return 3; // sentinel must read 0 once the data is there

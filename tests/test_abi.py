@@ -74,3 +74,11 @@ def test_noalloc_fft_refuses_oversized_scratch_on_the_host():
     assert L.msosm_k_fft_c2c_noalloc(p, p, 1 << 23, 1, 1 << 23, 1 << 23, 1, None) == ms.ERR_ARG
     assert L.msosm_k_fft_c2c_noalloc(p, p, 3000, 1, 3000, 3000, 1, None) == ms.ERR_ARG
     assert L.msosm_k_fft_c2c_noalloc(p, p, 1024, 1, 1024, 1024, 0, None) == ms.ERR_ARG
+
+
+def test_single_bf16_hankel_form_is_refused():
+    # single BF16 operands miss the 1e-3 mixed-precision gate: the engine
+    # refuses the form before touching a device
+    p = ms.Plan(64e6, [10.0], 1368e6, fftpoint=16384, count=2, form=ms.FORM_HANKEL_BF16)
+    with pytest.raises(ms.MsosmError, match="1e-3"):
+        ms.Engine(p)

@@ -16,22 +16,21 @@ MULTI = (64e6, [0.0, 0.4, 3.0, 7.5, 12.0, 18.25, 24.0, 29.0], 1368e6)
 
 
 def test_operand_forms_vs_fp64(device):
-    """FP16 operands within the mixed gate; BF16 hi+lo about 2^-16 relative;
-    single BF16 operands (2^-8 relative rounding) outside the 1e-3 budget,
-    which is why the split form exists."""
+    """FP16 operands within the mixed gate; BF16 hi+lo about 2^-16 relative.
+    (Single BF16 operands, 2^-8 relative rounding, miss the 1e-3 budget --
+    which is why the split form exists: the engine refuses that form, and
+    test_gemm_kernel_vs_numpy shows its GEMM error.)"""
     bw, dms, f0 = MULTI
     u = make_input(16 * 8192, 99)
     refs = [oracle_out(bw, dm, f0, 8192, u) for dm in dms]
     errs = {}
-    for form in (ms.FORM_HANKEL_F16, ms.FORM_HANKEL_BF16, ms.FORM_HANKEL_BF16X3):
+    for form in (ms.FORM_HANKEL_F16, ms.FORM_HANKEL_BF16X3):
         print(f"form {form}", flush=True)
         y = run_engine(ms.Plan(bw, dms, f0, fftpoint=16384, count=4, form=form), u, device=device)
         errs[form] = max(rel_l2(y[i], refs[i]) for i in range(len(dms)))
     print("max rel L2 vs FP64:", errs)
     assert errs[ms.FORM_HANKEL_F16] < 1e-3
     assert errs[ms.FORM_HANKEL_BF16X3] < 5e-5
-    assert 1e-3 < errs[ms.FORM_HANKEL_BF16] < 2e-2
-    assert errs[ms.FORM_HANKEL_BF16X3] < errs[ms.FORM_HANKEL_BF16] / 30
 
 
 def test_equals_gather_form(device):
