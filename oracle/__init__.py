@@ -61,6 +61,8 @@ def lib():
     L.ora_msosm_run.argtypes = [vp, sz, ul, vp, vp, vp]
     L.ora_msosm_run_blocks.restype = i
     L.ora_msosm_run_blocks.argtypes = [vp, sz, ul, vp, vp, sz, sz, vp]
+    L.ora_msosm_run_blocks_multi.restype = i
+    L.ora_msosm_run_blocks_multi.argtypes = [vp, sz, ul, i, vp, vp, sz, sz, vp]
     L.ora_cpu_msosm_f32.restype = ctypes.c_double
     L.ora_cpu_msosm_f32.argtypes = [vp, sz, ul, i, vp, vp, vp, i]
     L.ora_fold_create.restype = vp
@@ -179,6 +181,24 @@ def msosm_run(x, M, delay, H, b0=0, nb=None):
     rc = lib().ora_msosm_run_blocks(_p(x), L, M, _p(d), _p(Hc), b0, nb, _p(y))
     if rc != 0:
         raise ValueError("ora_msosm_run_blocks: bad sizes")
+    return y
+
+
+def msosm_run_multi(x, M, delays, H, b0=0, nb=None):
+    """Output blocks b0 .. b0+nb-1 for every trial (rows of delays [ndm][2M]
+    and H [ndm][2M]) of one stream x; returns [ndm][nb*M] complex128."""
+    x = np.ascontiguousarray(x, np.complex128)
+    L = x.size
+    if nb is None:
+        nb = L // M - b0
+    d = np.ascontiguousarray(delays, np.int32)
+    Hc = np.ascontiguousarray(H, np.complex128)
+    ndm = d.shape[0]
+    assert d.shape == (ndm, 2 * M) and Hc.shape == (ndm, 2 * M)
+    y = np.zeros((ndm, nb * M), np.complex128)
+    rc = lib().ora_msosm_run_blocks_multi(_p(x), L, M, ndm, _p(d), _p(Hc), b0, nb, _p(y))
+    if rc != 0:
+        raise ValueError("ora_msosm_run_blocks_multi: bad sizes")
     return y
 
 

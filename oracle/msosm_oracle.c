@@ -30,6 +30,8 @@
  *   ora_fold_* -- crafted periods with known fractional parts (skip every
  *       1/frac periods), periodic impulse trains landing in known bins,
  *       split-invariance of the state machine, numpy restatement fixtures.
+ *   ora_msosm_run_blocks_multi -- equal bit for bit, trial by trial, to
+ *       ora_msosm_run_blocks.
  *   ora_cpu_msosm_f32 -- against ora_msosm_run (FP64) within FP32 rounding.
  */
 #include "msosm_oracle.h"
@@ -386,6 +388,84 @@ int ora_msosm_run_blocks(const double *x, size_t L, unsigned long M, const int32
 int ora_msosm_run(const double *x, size_t L, unsigned long M, const int32_t *dp, const double *H, double *y)
 {
     return ora_msosm_run_blocks(x, L, M, dp, H, 0, L / M, y);
+}
+
+/* The same for ndm trials of one stream (msosm_gpu_dm_concurrent.cpp:119-151:
+ * one forward spectrum per block shared by every trial): output blocks b0 ..
+ * b0+nb-1 of trial t go to y[t][nb*M], delays dp[t][2M], filters H[t][2M].
+ * The arithmetic per trial is that of ora_msosm_run_blocks operation for
+ * operation (same window FFTs, same products, same inverse), so each row
+ * equals ora_msosm_run_blocks for that trial bit for bit; trials run in
+ * parallel. */
+int ora_msosm_run_blocks_multi(const double *x, size_t L, unsigned long M, int ndm, const int32_t *dp,
+                               const double *H, size_t b0, size_t nb, double *y)
+{
+    if (M == 0 || L % M != 0 || b0 + nb > L / M || ndm < 1)
+        return -1;
+    const size_t N = 2 * M;
+    int K = 1;
+    for (size_t i = 0; i < (size_t)ndm * N; i++)
+        if (dp[i] + 1 > K)
+            K = dp[i] + 1;
+    /* spectra of blocks gstart .. b0+nb-1, where gstart is the first block
+     * any trial reads: the single-trial routine starts at b0 - (K_t - 1) and
+     * treats blocks before that as zero, so each trial below does the same
+     * with its own K_t */
+    const size_t gstart = (b0 >= (size_t)(K - 1)) ? b0 - (size_t)(K - 1) : 0;
+    const size_t nw = b0 + nb - gstart;
+    double *spec = (double *)malloc(sizeof(double) * 2 * N * nw);
+    if (!spec)
+        return -2;
+#pragma omp parallel for schedule(dynamic)
+    for (long long w = 0; w < (long long)nw; w++) {
+        const size_t g = gstart + (size_t)w;
+        double *X = spec + (size_t)w * 2 * N;
+        for (size_t s = 0; s < N; s++) {
+            long long idx = (long long)(g * M + s) - (long long)M;
+            if (idx >= 0 && (size_t)idx < L) {
+                X[2 * s] = x[2 * idx];
+                X[2 * s + 1] = x[2 * idx + 1];
+            } else {
+                X[2 * s] = 0;
+                X[2 * s + 1] = 0;
+            }
+        }
+        ora_fft(X, N, -1);
+    }
+#pragma omp parallel for schedule(dynamic)
+    for (int t = 0; t < ndm; t++) {
+        const int32_t *d = dp + (size_t)t * N;
+        const double *Ht = H + (size_t)t * 2 * N;
+        int Kt = 1;
+        for (size_t n = 0; n < N; n++)
+            if (d[n] + 1 > Kt)
+                Kt = d[n] + 1;
+        const size_t gs = (b0 >= (size_t)(Kt - 1)) ? b0 - (size_t)(Kt - 1) : 0;
+        double *Y = (double *)malloc(sizeof(double) * 2 * N);
+        for (size_t g = b0; g < b0 + nb; g++) {
+            for (size_t n = 0; n < N; n++) {
+                long long src = (long long)g - d[n];
+                if (src < (long long)gs) {
+                    Y[2 * n] = 0;
+                    Y[2 * n + 1] = 0;
+                    continue;
+                }
+                const double *Xs = spec + ((size_t)src - gstart) * 2 * N;
+                double ar = Xs[2 * n] / (double)N, ai = Xs[2 * n + 1] / (double)N;
+                Y[2 * n] = ar * Ht[2 * n] - ai * Ht[2 * n + 1];
+                Y[2 * n + 1] = ar * Ht[2 * n + 1] + ai * Ht[2 * n];
+            }
+            ora_fft(Y, N, +1);
+            double *yt = y + (size_t)t * 2 * nb * M;
+            for (size_t o = 0; o < M; o++) {
+                yt[2 * ((g - b0) * M + o)] = Y[2 * (M + o)] / (double)N;
+                yt[2 * ((g - b0) * M + o) + 1] = Y[2 * (M + o) + 1] / (double)N;
+            }
+        }
+        free(Y);
+    }
+    free(spec);
+    return 0;
 }
 
 /* ------------------------------------------------------------------------ */

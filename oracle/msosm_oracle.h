@@ -54,6 +54,10 @@ int ora_msosm_run(const double *x, size_t L, unsigned long M, const int32_t *del
                   const double *H, double *y);
 /* Same, for a contiguous sub-range of output blocks [b0, b0+nb) only; x must
  * hold the stream from sample 0 up to (b0+nb)*M. */
+/* ora_msosm_run_blocks for ndm trials sharing the forward spectra:
+ * delay_points[ndm][2M], H[ndm][2M] complex, y[ndm][nb*M] complex. */
+int ora_msosm_run_blocks_multi(const double *x, size_t L, unsigned long M, int ndm, const int32_t *delay_points,
+                               const double *H, size_t b0, size_t nb, double *y);
 int ora_msosm_run_blocks(const double *x, size_t L, unsigned long M, const int32_t *delay_points,
                          const double *H, size_t b0, size_t nb, double *y);
 
