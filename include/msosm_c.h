@@ -213,6 +213,17 @@ int msosm_k_discard_batch(const float *a, float *b, int M, int count, int num_dm
  * forward, +1 inverse (unnormalised).  in and out must not overlap. */
 int msosm_k_fft_c2c(const float *in, float *out, size_t n, size_t batch, size_t idist, size_t odist,
                     int direction, void *stream);
+/* FFT plans (cufftPlanMany / cufftExecC2C / cufftDestroy, e.g.
+ * msosm_gpu_batch.cpp:41-49,84,91): a plan fixes the size, batch and row
+ * strides, and owns the two-pass scratch (sizes above 2^14 points) that
+ * msosm_k_fft_c2c allocates per call; exec transforms in[b*idist + k] to
+ * out[b*odist + k], direction -1 forward, +1 inverse (unnormalised).  One
+ * plan's execs share its scratch: run them on one stream (or wait between
+ * them). */
+typedef struct msosm_fft_plan msosm_fft_plan;
+int msosm_fft_plan_create(size_t n, size_t batch, size_t idist, size_t odist, msosm_fft_plan **out);
+int msosm_fft_plan_exec(msosm_fft_plan *plan, const float *in, float *out, int direction, void *stream);
+int msosm_fft_plan_destroy(msosm_fft_plan *plan);
 /* As msosm_k_fft_c2c, allocating nothing.  No counterpart in the reference:
  * it is for callers that cannot allocate device memory (code under graph
  * capture, serving loops, harnesses that forbid hipMalloc).  Sizes above
@@ -332,6 +343,19 @@ int msosm_rows_standardize(float *x, int rows, int T, void *stream);
 /* out[k][t] = sum_c S[c][t + delay[k][c]] / sqrt(nchan), t < Tout. */
 int msosm_incoherent_dedisperse(const float *S, int nchan, int T, const int *delay, int K, int Tout, float *out,
                                 void *stream);
+/* out[r][b] = scale * mean of x[r][t] over samples t with floor(frac(t /
+ * period) * nbins) = b (period in samples, FP64 phase); 0 where no sample
+ * falls. */
+int msosm_rows_fold(const float *x, int rows, int T, double period, int nbins, float scale, float *out,
+                    void *stream);
+/* Folded profiles of K trials from channels folded at `period` seconds:
+ * out[k][b] = sum_c Sf[c][(b + rint(delay_s[k][c] / period * nbins)) mod
+ * nbins] / sqrt(nchan) (delay_s in seconds). */
+int msosm_incoherent_fold(const float *Sf, int nchan, int nbins, const float *delay_s, int K, double period,
+                          float *out, void *stream);
+/* out[k][r] = |DFT_nbins(prof[r])[k + 1]| / sqrt(nbins), k < K <= nbins/2:
+ * harmonic amplitudes of folded profiles, harmonic-major. */
+int msosm_rows_harmonics(const float *prof, int rows, int nbins, int K, float *out, void *stream);
 int msosm_conv1d_fwd(const float *x, const void *w16, const float *bias, float *y, int B, int Cin, int Cout, int T,
                      int KW, int dil, int relu, void *stream);
 /* gx (may be NULL) = data gradient; gw, gb are accumulated (+=). */

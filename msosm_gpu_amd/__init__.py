@@ -108,6 +108,9 @@ SIGNATURES = {
     "msosm_k_discard_batch": (_i, [_vp, _vp, _i, _i, _i, _vp]),
     "msosm_k_fft_c2c": (_i, [_vp, _vp, _sz, _sz, _sz, _sz, _i, _vp]),
     "msosm_k_fft_c2c_noalloc": (_i, [_vp, _vp, _sz, _sz, _sz, _sz, _i, _vp]),
+    "msosm_fft_plan_create": (_i, [_sz, _sz, _sz, _sz, ctypes.POINTER(_vp)]),
+    "msosm_fft_plan_exec": (_i, [_vp, _vp, _vp, _i, _vp]),
+    "msosm_fft_plan_destroy": (_i, [_vp]),
     "msosm_host_alloc": (_i, [_sz, ctypes.POINTER(_vp)]),
     "msosm_refsim_info": (_i, [_f, _f, _f, _f, ctypes.POINTER(_ul), ctypes.POINTER(_ul), ctypes.POINTER(_ul),
                                ctypes.POINTER(_ul)]),
@@ -147,6 +150,9 @@ SIGNATURES = {
     "msosm_spectrogram_u16": (_i, [_vp, _sz, _i, _i, _vp, _vp]),
     "msosm_rows_standardize": (_i, [_vp, _i, _i, _vp]),
     "msosm_incoherent_dedisperse": (_i, [_vp, _i, _i, _vp, _i, _i, _vp, _vp]),
+    "msosm_rows_harmonics": (_i, [_vp, _i, _i, _i, _vp, _vp]),
+    "msosm_incoherent_fold": (_i, [_vp, _i, _i, _vp, _i, ctypes.c_double, _vp, _vp]),
+    "msosm_rows_fold": (_i, [_vp, _i, _i, ctypes.c_double, _i, _f, _vp, _vp]),
     "msosm_conv1d_fwd": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _i, _i, _vp]),
     "msosm_conv1d_bwd": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _i, _i, _vp]),
     "msosm_pool_fwd": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp]),
@@ -466,6 +472,28 @@ def fft_c2c(inp, out, n, batch, idist=None, odist=None, direction=-1, stream=Non
     idist = n if idist is None else idist
     odist = n if odist is None else odist
     check(lib().msosm_k_fft_c2c(_ptr(inp), _ptr(out), n, batch, idist, odist, direction, _stream_handle(stream)))
+
+
+class FftPlan:
+    """Batched C2C FFT plan owning its two-pass scratch (msosm_fft_plan_*)."""
+
+    def __init__(self, n, batch=1, idist=None, odist=None):
+        h = ctypes.c_void_p()
+        check(lib().msosm_fft_plan_create(n, batch, n if idist is None else idist, n if odist is None else odist,
+                                          ctypes.byref(h)))
+        self.h = h
+        self.n, self.batch = n, batch
+
+    def __call__(self, inp, out, direction=-1, stream=None):
+        check(lib().msosm_fft_plan_exec(self.h, _ptr(inp), _ptr(out), direction, _stream_handle(stream)))
+
+    def close(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.msosm_fft_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
 
 
 def refsim_info(bw, dm, f0, period):

@@ -1162,6 +1162,58 @@ int msosm_k_inverse_discard(const void *Y, float *out, int y_f16, int log2n, siz
     return MSOSM_OK;
 }
 
+struct msosm_fft_plan {
+    int log2n = 0;
+    size_t batch = 0, idist = 0, odist = 0;
+    float2 *scratch = nullptr;
+};
+
+int msosm_fft_plan_create(size_t n, size_t batch, size_t idist, size_t odist, msosm_fft_plan **out)
+{
+    if (!out || n < 2 || (n & (n - 1)) || n > (1ul << 28) || batch < 1 || idist < 1 || odist < 1)
+        return set_error(MSOSM_ERR_ARG, "fft_plan_create: n must be a power of two in [2, 2^28], batch >= 1");
+    *out = nullptr;
+    msosm_fft_plan *p = new msosm_fft_plan();
+    p->log2n = ilog2_exact(n);
+    p->batch = batch;
+    p->idist = idist;
+    p->odist = odist;
+    const size_t sc = msosm::fft_c2c_scratch_elems(p->log2n, (long long)batch);
+    if (sc) {
+        hipError_t e = dev_alloc(&p->scratch, sc);
+        if (e != hipSuccess) {
+            delete p;
+            return set_error(MSOSM_ERR_ALLOC, "fft_plan_create: scratch of %zu bytes: %s", sc * sizeof(float2),
+                             hipGetErrorString(e));
+        }
+    }
+    *out = p;
+    return MSOSM_OK;
+}
+
+int msosm_fft_plan_exec(msosm_fft_plan *p, const float *in, float *out, int direction, void *stream)
+{
+    if (!p || !in || !out || (direction != -1 && direction != 1))
+        return set_error(MSOSM_ERR_ARG, "fft_plan_exec: bad argument (direction -1 or +1)");
+    HIP_TRY(msosm::launch_fft_c2c((const float2 *)in, (float2 *)out, p->log2n, (long long)p->batch,
+                                  (long long)p->idist, (long long)p->odist, direction == 1, 1.0f, p->scratch,
+                                  (hipStream_t)stream));
+    return MSOSM_OK;
+}
+
+int msosm_fft_plan_destroy(msosm_fft_plan *p)
+{
+    if (!p)
+        return MSOSM_OK;
+    if (p->scratch) {
+        // execs may still be in flight on any stream
+        HIP_TRY(hipDeviceSynchronize());
+        HIP_TRY(hipFree(p->scratch));
+    }
+    delete p;
+    return MSOSM_OK;
+}
+
 int msosm_k_fft_c2c(const float *in, float *out, size_t n, size_t batch, size_t idist, size_t odist, int direction,
                     void *stream)
 {

@@ -205,6 +205,82 @@ __global__ void k_incoherent(const float *__restrict__ S, int nchan, int T, cons
     out[(size_t)k * Tout + t] = s * rsqrtf((float)nchan);
 }
 
+// out[r][b] = scale * mean of x[r][t] over the t whose phase frac(t / period)
+// falls in bin b of nbins (FP64 phase), 0 for a bin no sample reaches.  One
+// workgroup per row, sums and counts in LDS.
+__global__ __launch_bounds__(256) void k_rows_fold(const float *__restrict__ x, int T, double inv_period, int nbins,
+                                                   float scale, float *__restrict__ out)
+{
+    extern __shared__ float fold_lds[];
+    float *sum = fold_lds, *cnt = fold_lds + nbins;
+    for (int b = threadIdx.x; b < 2 * nbins; b += blockDim.x)
+        fold_lds[b] = 0.f;
+    __syncthreads();
+    const float *row = x + (size_t)blockIdx.x * T;
+    for (int t = threadIdx.x; t < T; t += blockDim.x) {
+        double ph = (double)t * inv_period;
+        ph -= floor(ph);
+        int b = (int)(ph * nbins);
+        b = b < nbins ? b : nbins - 1;
+        atomicAdd(&sum[b], row[t]);
+        atomicAdd(&cnt[b], 1.0f);
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < nbins; b += blockDim.x)
+        out[(size_t)blockIdx.x * nbins + b] = cnt[b] > 0.f ? scale * sum[b] / cnt[b] : 0.f;
+}
+
+// Incoherent dedispersion of folded channels: with Sf[c][b] channel c folded
+// at the period into nbins phase bins, trial k's folded profile is
+// out[k][b] = sum_c Sf[c][(b + s) mod nbins] / sqrt(nchan), s = the channel's
+// delay delay_s[k][c] (seconds) as a whole number of phase bins,
+// rint(delay / period * nbins) (FP64).  Folding is linear, so this is the
+// fold of the incoherently dedispersed series up to that rounding.  One
+// thread per (trial, bin).
+__global__ void k_incoherent_fold(const float *__restrict__ Sf, int nchan, int nbins,
+                                  const float *__restrict__ delay_s, double inv_period, float *__restrict__ out)
+{
+    const int k = blockIdx.y;
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nbins)
+        return;
+    const float *dk = delay_s + (size_t)k * nchan;
+    float s = 0.f;
+    for (int c = 0; c < nchan; c++) {
+        const double sh = rint((double)dk[c] * inv_period * nbins);
+        long long q = (long long)sh % nbins;
+        q += q < 0 ? nbins : 0;
+        int i = b + (int)q;
+        i -= i >= nbins ? nbins : 0;
+        s += Sf[(size_t)c * nbins + i];
+    }
+    out[(size_t)k * nbins + b] = s * rsqrtf((float)nchan);
+}
+
+// out[k][r] = |sum_b p[r][b] exp(-2 pi i (k+1) b / nbins)| / sqrt(nbins) for
+// k < K: amplitudes of the first K harmonics of each row's folded profile
+// (independent of where in phase the pulse sits), harmonic-major so that the
+// rows become the time axis of a convolution over trials.  One thread per
+// (row, harmonic); the angle is reduced exactly as ((k+1) b mod nbins).
+__global__ void k_rows_harmonics(const float *__restrict__ p, int R, int nbins, int K, float *__restrict__ out)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = blockIdx.y;
+    if (r >= R)
+        return;
+    const float *row = p + (size_t)r * nbins;
+    float re = 0.f, im = 0.f;
+    const int h = k + 1;
+    for (int b = 0; b < nbins; b++) {
+        const int e = (h * b) % nbins;
+        float s, c;
+        sincospif(2.0f * (float)e / (float)nbins, &s, &c);
+        re += row[b] * c;
+        im -= row[b] * s;
+    }
+    out[(size_t)k * R + r] = sqrtf(re * re + im * im) * rsqrtf((float)nbins);
+}
+
 // ---------------------------------------------------------------------------
 // 1D convolution, same padding: y[b][co][t] = bias[co] +
 //   sum_ci sum_k w[co][ci][k] x[b][ci][t + dil (k - KW/2)]   (ReLU optional).
@@ -573,6 +649,38 @@ int msosm_incoherent_dedisperse(const float *S, int nchan, int T, const int *del
         return msosm::set_error(MSOSM_ERR_ARG, "incoherent_dedisperse: bad argument");
     hipLaunchKernelGGL(k_incoherent, dim3((Tout + 255) / 256, K), dim3(256), 0, (hipStream_t)stream, S, nchan, T,
                        delay, Tout, out);
+    LHIP(hipGetLastError());
+    return MSOSM_OK;
+}
+
+int msosm_rows_fold(const float *x, int rows, int T, double period, int nbins, float scale, float *out,
+                    void *stream)
+{
+    if (!x || !out || rows < 1 || T < 1 || !(period > 0) || nbins < 1 || nbins > 4096)
+        return msosm::set_error(MSOSM_ERR_ARG, "rows_fold: bad argument (period > 0, 1 <= nbins <= 4096)");
+    hipLaunchKernelGGL(k_rows_fold, dim3(rows), dim3(256), sizeof(float) * 2 * nbins, (hipStream_t)stream, x, T,
+                       1.0 / period, nbins, scale, out);
+    LHIP(hipGetLastError());
+    return MSOSM_OK;
+}
+
+int msosm_incoherent_fold(const float *Sf, int nchan, int nbins, const float *delay_s, int K, double period,
+                          float *out, void *stream)
+{
+    if (!Sf || !delay_s || !out || nchan < 1 || nbins < 1 || K < 1 || K > 65535 || !(period > 0))
+        return msosm::set_error(MSOSM_ERR_ARG, "incoherent_fold: bad argument");
+    hipLaunchKernelGGL(k_incoherent_fold, dim3((nbins + 63) / 64, K), dim3(64), 0, (hipStream_t)stream, Sf, nchan,
+                       nbins, delay_s, 1.0 / period, out);
+    LHIP(hipGetLastError());
+    return MSOSM_OK;
+}
+
+int msosm_rows_harmonics(const float *prof, int rows, int nbins, int K, float *out, void *stream)
+{
+    if (!prof || !out || rows < 1 || nbins < 2 || K < 1 || K > nbins / 2 || K > 65535)
+        return msosm::set_error(MSOSM_ERR_ARG, "rows_harmonics: bad argument (1 <= K <= nbins / 2)");
+    hipLaunchKernelGGL(k_rows_harmonics, dim3((rows + 255) / 256, K), dim3(256), 0, (hipStream_t)stream, prof, rows,
+                       nbins, K, out);
     LHIP(hipGetLastError());
     return MSOSM_OK;
 }

@@ -20,7 +20,7 @@ import math
 
 import numpy as np
 
-from . import (CHIRP_REF_F32W, FORM_FUSED_F32, Engine, Plan, _ptr, check, fft_c2c, fold_phase, lib)
+from . import (CHIRP_REF_F32W, FORM_FUSED_F32, Engine, FftPlan, Plan, _ptr, check, fold_phase, lib)
 
 KDM = 4.15e15
 
@@ -39,40 +39,80 @@ def snr_of_profile(prof, hits):
 
 
 class Frontend:
-    """Spectrogram -> incoherent dedispersion to n_coarse DM rows."""
+    """Spectrogram -> incoherent dedispersion to DM rows (n_coarse trials on
+    [0, dm_max], or the trials `dms`).  Without fold_bins the rows are
+    standardised time series [trials][time].  With fold_bins the trials'
+    series are instead folded at the pulsar's period (known to the search, as
+    to the folding judge) into fold_bins phase bins at unit noise -- computed
+    by folding each channel and shifting it by its dispersion delay in phase
+    bins, which is the same linear map -- giving [trials][phase]; with
+    harmonics as well, each folded profile becomes the amplitudes of its first
+    `harmonics` harmonics (independent of the pulse phase), laid out
+    [harmonics][trials] so that a convolution runs along the trial axis."""
 
-    def __init__(self, bw, f0, dm_max, n_coarse, n_samples, device, nchan=256, navg=16):
+    def __init__(self, bw, f0, dm_max, n_coarse, n_samples, device, nchan=256, navg=16, fold_bins=0, harmonics=0,
+                 dms=None):
         import torch
         self.bw, self.f0, self.nchan, self.navg = bw, f0, nchan, navg
+        self.fold_bins, self.harmonics = fold_bins, harmonics
+        if harmonics and not fold_bins:
+            raise ValueError("harmonic features need folded rows (fold_bins)")
         self.T = n_samples // (nchan * navg)
         self.tbin = nchan * navg / bw
-        self.dms = np.linspace(0.0, dm_max, n_coarse)
+        self.dms = np.linspace(0.0, dm_max, n_coarse) if dms is None else np.asarray(dms, np.float64)
         fc = f0 + np.arange(nchan) * (bw / nchan)          # channel c -> offset c*bw/nchan
         ftop = fc[-1]
-        d = np.rint(KDM * self.dms[:, None] * (1 / fc[None, :] ** 2 - 1 / ftop ** 2) / self.tbin).astype(np.int32)
+        delay_s = KDM * self.dms[:, None] * (1 / fc[None, :] ** 2 - 1 / ftop ** 2)
+        d = np.rint(delay_s / self.tbin).astype(np.int32)
         self.delay_host = d
-        self.Tout = self.T - int(d.max())
-        if self.Tout < 64:
+        self.Tfull = self.T - int(d.max())
+        self.Tout = len(self.dms) if harmonics else (fold_bins if fold_bins else self.Tfull)
+        # network input [channels][steps]: trials x time, trials x phase, or
+        # harmonics x trials
+        self.shape = (harmonics, len(self.dms)) if harmonics else (len(self.dms), self.Tout)
+        if self.Tfull < 64:
             raise ValueError("input too short for the DM range")
         self.delay = torch.from_numpy(d).to(device)
+        self.delay_s = torch.from_numpy(delay_s.astype(np.float32)).to(device)
         self.S = torch.empty((nchan, self.T), dtype=torch.float32, device=device)
+        self.Sf = torch.empty((nchan, fold_bins), dtype=torch.float32, device=device) if fold_bins else None
+        self.prof = (torch.empty((len(self.dms), fold_bins), dtype=torch.float32, device=device) if harmonics
+                     else None)
         self.device = device
 
-    def __call__(self, u16_dev, out):
-        """u16_dev: packed input (T*nchan*navg samples); out: [n_coarse][Tout]."""
+    def __call__(self, u16_dev, out, period_s=None):
+        """u16_dev: packed input (T*nchan*navg samples); out: [n_coarse][Tout].
+        period_s: the pulsar's period (needed with fold_bins)."""
         n = self.T * self.nchan * self.navg
         L = lib()
         check(L.msosm_spectrogram_u16(_p(u16_dev), n, self.nchan, self.navg, _p(self.S), None))
         check(L.msosm_rows_standardize(_p(self.S), self.nchan, self.T, None))
+        if self.fold_bins:
+            if period_s is None:
+                raise ValueError("a folding front end needs the period")
+            check(L.msosm_rows_fold(_p(self.S), self.nchan, self.T, period_s / self.tbin, self.fold_bins,
+                                    math.sqrt(self.T / self.fold_bins), _p(self.Sf), None))
+            prof = self.prof if self.harmonics else out
+            check(L.msosm_incoherent_fold(_p(self.Sf), self.nchan, self.fold_bins, _p(self.delay_s), len(self.dms),
+                                          period_s, _p(prof), None))
+            if self.harmonics:
+                check(L.msosm_rows_harmonics(_p(prof), len(self.dms), self.fold_bins, self.harmonics, _p(out), None))
+            return
         check(L.msosm_incoherent_dedisperse(_p(self.S), self.nchan, self.T, _p(self.delay), len(self.dms),
                                             self.Tout, _p(out), None))
         check(L.msosm_rows_standardize(_p(out), len(self.dms), self.Tout, None))
 
 
 class DMNet:
-    def __init__(self, c_in, n_out, width=64, kw=9, dils=(1, 2, 4), seed=0, device="cuda:0"):
+    """Dilated 1D convolutions (ReLU) followed by either head "pool" (max and
+    mean over time, a linear layer to n_out logits) or head "conv" (a final
+    one-channel convolution: the time axis is the trial axis and its values
+    are the logits, n_out = the input's length)."""
+
+    def __init__(self, c_in, n_out, width=64, kw=9, dils=(1, 2, 4), seed=0, device="cuda:0", head="pool"):
         import torch
         self.c_in, self.n_out, self.width, self.kw, self.dils = c_in, n_out, width, kw, tuple(dils)
+        self.head = head
         g = torch.Generator().manual_seed(seed)
         self.shapes = {}
         c = c_in
@@ -80,8 +120,12 @@ class DMNet:
             self.shapes[f"conv{i}_w"] = (width, c, kw)
             self.shapes[f"conv{i}_b"] = (width,)
             c = width
-        self.shapes["fc_w"] = (n_out, 2 * width)
-        self.shapes["fc_b"] = (n_out,)
+        if head == "conv":
+            self.shapes["head_w"] = (1, width, kw)
+            self.shapes["head_b"] = (1,)
+        else:
+            self.shapes["fc_w"] = (n_out, 2 * width)
+            self.shapes["fc_b"] = (n_out,)
         self.p, self.p16, self.g, self.m, self.v = {}, {}, {}, {}, {}
         for k, s in self.shapes.items():
             if k.endswith("_w"):
@@ -116,6 +160,13 @@ class DMNet:
                                      h.shape[1], self.width, T, self.kw, d, 1, None))
             self.acts.append(y)
             h = y
+        if self.head == "conv":
+            if T != self.n_out:
+                raise ValueError(f"the convolutional head gives one logit per input step ({T}), not {self.n_out}")
+            self.hy = torch.empty((B, 1, T), dtype=torch.float32, device=self.device)
+            check(L.msosm_conv1d_fwd(_p(h), _p(self.p16["head_w"]), _p(self.p["head_b"]), _p(self.hy), B, self.width,
+                                     1, T, self.kw, 1, 0, None))
+            return self.hy.view(B, T)
         self.feat = torch.empty((B, 2 * self.width), dtype=torch.float32, device=self.device)
         self.amax = torch.empty((B, self.width), dtype=torch.int32, device=self.device)
         check(L.msosm_pool_fwd(_p(h), B, self.width, T, _p(self.feat), _p(self.amax), None))
@@ -131,11 +182,17 @@ class DMNet:
             t.zero_()
         B = glogits.shape[0]
         T = self.acts[0].shape[2]
-        gfeat = torch.empty_like(self.feat)
-        check(L.msosm_linear_bwd(_p(self.feat), _p(self.p16["fc_w"]), _p(glogits), _p(gfeat), _p(self.g["fc_w"]),
-                                 _p(self.g["fc_b"]), B, 2 * self.width, self.n_out, None))
         gh = torch.empty_like(self.acts[-1])
-        check(L.msosm_pool_bwd(_p(gfeat), _p(self.amax), B, self.width, T, _p(gh), None))
+        if self.head == "conv":
+            gl = glogits.contiguous()
+            check(L.msosm_conv1d_bwd(_p(self.acts[-1]), _p(self.p16["head_w"]), _p(self.hy), _p(gl), _p(gh),
+                                     _p(self.g["head_w"]), _p(self.g["head_b"]), B, self.width, 1, T, self.kw, 1, 0,
+                                     None))
+        else:
+            gfeat = torch.empty_like(self.feat)
+            check(L.msosm_linear_bwd(_p(self.feat), _p(self.p16["fc_w"]), _p(glogits), _p(gfeat),
+                                     _p(self.g["fc_w"]), _p(self.g["fc_b"]), B, 2 * self.width, self.n_out, None))
+            check(L.msosm_pool_bwd(_p(gfeat), _p(self.amax), B, self.width, T, _p(gh), None))
         for i in reversed(range(len(self.dils))):
             x, y = self.acts[i], self.acts[i + 1]
             gx = torch.empty_like(x) if (i > 0 or want_input_grad) else None
@@ -171,6 +228,17 @@ class DMNet:
 
 
 # ---------------------------------------------------------------------------
+_fft_plans = {}
+
+
+def _fft_plan(n):
+    """One cached whole-signal FFT plan per length (its scratch is allocated
+    once, not per transform)."""
+    if n not in _fft_plans:
+        _fft_plans[n] = FftPlan(n)
+    return _fft_plans[n]
+
+
 def simulate(n, bw, f0, dm, period_s, duty, snr, seed, device, scale=64.0):
     """Packed uint16 baseband (device tensor) with a dispersed pulsar: noise
     power 1, period-averaged pulsar power snr (see msosm_sim_* kernels)."""
@@ -181,10 +249,11 @@ def simulate(n, bw, f0, dm, period_s, duty, snr, seed, device, scale=64.0):
     sigma = duty / 2.3548
     gmean = sigma * math.sqrt(2 * math.pi) * math.erf(0.5 / (sigma * math.sqrt(2)))
     amp = snr / max(gmean, 1e-12)
+    plan = _fft_plan(n)
     check(L.msosm_sim_pulsar(_p(x), n, period_s * bw, duty, amp, seed & 0xFFFFFFFF, None))
-    fft_c2c(x, X, n, 1, direction=-1)
+    plan(x, X, direction=-1)
     check(L.msosm_sim_disperse(_p(X), n, bw, f0, dm, +1, None))
-    fft_c2c(X, x, n, 1, direction=+1)
+    plan(X, x, direction=+1)
     u = torch.empty(n, dtype=torch.int32, device=device)
     check(L.msosm_sim_quantize(_p(x), n, 1.0 / n, (seed * 2654435761) & 0xFFFFFFFF, scale, _p(u), None))
     return u
@@ -203,13 +272,17 @@ def power_for_snr(target_snr, n, duty, nbins=64):
 
 class Judge:
     """Plain coherent method as judge: dedisperse at a set of trials of a fixed
-    fine DM grid (engine, FP32 fused form) and phase-fold every call's output;
-    engines are cached per trial range so repeated calls only launch kernels."""
+    fine DM grid (engine, FP32 fused form unless another form is given) and
+    phase-fold every call's output; engines are cached per trial range (the
+    max_engines most recently used) so repeated calls only launch kernels."""
 
-    def __init__(self, bw, f0, fine_dms, n_samples, fftpoint=16384, count=32, nbins=64, device="cuda:0"):
+    def __init__(self, bw, f0, fine_dms, n_samples, fftpoint=16384, count=32, nbins=64, device="cuda:0",
+                 form=FORM_FUSED_F32, max_engines=None):
+        from collections import OrderedDict
         self.bw, self.f0, self.fine = bw, f0, np.asarray(fine_dms, np.float32)
         self.n, self.fftpoint, self.count, self.nbins, self.device = n_samples, fftpoint, count, nbins, device
-        self.engines = {}
+        self.form, self.max_engines = form, max_engines
+        self.engines = OrderedDict()
         self.trials_run = 0
         # Fold only output past the longest filter transient of the grid (the
         # first K-1 blocks of an engine ramp up from zero history, which would
@@ -222,9 +295,14 @@ class Judge:
 
     def _engine(self, lo, hi):
         key = (lo, hi)
-        if key not in self.engines:
+        if key in self.engines:
+            self.engines.move_to_end(key)
+        else:
+            if self.max_engines is not None:
+                while len(self.engines) >= self.max_engines:
+                    self.engines.popitem(last=False)[1].close()
             plan = Plan(self.bw, self.fine[lo:hi], self.f0, fftpoint=self.fftpoint, count=self.count,
-                        form=FORM_FUSED_F32, chirp=CHIRP_REF_F32W)
+                        form=self.form, chirp=CHIRP_REF_F32W)
             self.engines[key] = Engine(plan)
         return self.engines[key]
 
@@ -302,24 +380,24 @@ def draw_pulsars(n_examples, dm_max, period_range, duty_range, snr_range, seed):
 
 
 def make_dataset(pulsars, n_samples, bw, f0, frontend, judge, label_halfwidth=8, label_smooth=0, nbins=64,
-                 device="cuda:0", progress=None):
+                 device="cuda:0", progress=None, label_stride=32):
     """Simulate each pulsar, run the front end, and label it with the plain
     coherent method: the fine trial with the highest folded S/N within
     +-label_halfwidth trials of the true DM, after a running mean over
     +-label_smooth trials (the S/N of neighbouring trials differs mostly by
     noise when the pulse is wide compared with their smearing difference)."""
     import torch
-    X = torch.empty((len(pulsars), len(frontend.dms), frontend.Tout), dtype=torch.float32, device=device)
+    X = torch.empty((len(pulsars),) + frontend.shape, dtype=torch.float32, device=device)
     nf = len(judge.fine)
     labels = []
     for i, m in enumerate(pulsars):
         u = simulate(n_samples, bw, f0, m["dm"], m["period"], m["duty"],
                      power_for_snr(m["snr"], n_samples, m["duty"], nbins), m["seed"], device)
-        frontend(u, X[i])
+        frontend(u, X[i], m["period"])
         j0 = int(np.argmin(np.abs(judge.fine - m["dm"])))
-        # trial windows on a stride of 32 keep the number of cached engines small
-        width = min(nf, 2 * label_halfwidth + 32)
-        lo = min(max(0, (j0 - label_halfwidth) // 32 * 32), nf - width)
+        # trial windows on a fixed stride keep the number of cached engines small
+        width = min(nf, 2 * label_halfwidth + label_stride)
+        lo = min(max(0, (j0 - label_halfwidth) // label_stride * label_stride), nf - width)
         s = judge.snr_range(u, m["period"], lo, lo + width)
         if label_smooth > 0:
             ker = np.ones(2 * label_smooth + 1)
@@ -369,9 +447,8 @@ class Proposer:
 
     def search(self, u16_dev, period_s):
         import torch
-        x = torch.empty((1, len(self.frontend.dms), self.frontend.Tout), dtype=torch.float32,
-                        device=self.frontend.device)
-        self.frontend(u16_dev, x[0])
+        x = torch.empty((1,) + self.frontend.shape, dtype=torch.float32, device=self.frontend.device)
+        self.frontend(u16_dev, x[0], period_s)
         logits = self.net.forward(x).cpu().numpy()[0]
         idx = topk(logits, self.k)
         s = self.judge.snr_trials(u16_dev, period_s, idx)

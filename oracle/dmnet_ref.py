@@ -73,11 +73,15 @@ def softmax_xent(logits, target):
 
 
 class Net:
-    """params: dict of float64 arrays conv{i}_w, conv{i}_b, fc_w, fc_b."""
+    """params: dict of float64 arrays conv{i}_w, conv{i}_b and either fc_w,
+    fc_b (head "pool": max+mean pooling over time, linear layer) or head_w,
+    head_b (head "conv": a final one-channel convolution whose time axis is
+    the logits)."""
 
-    def __init__(self, params, dils):
+    def __init__(self, params, dils, head="pool"):
         self.p = params
         self.dils = dils
+        self.head = head
 
     def forward(self, x):
         cache = [x]
@@ -85,16 +89,24 @@ class Net:
         for i, d in enumerate(self.dils):
             h = conv1d_fwd(h, self.p[f"conv{i}_w"], self.p[f"conv{i}_b"], d, True)
             cache.append(h)
+        if self.head == "conv":
+            y = conv1d_fwd(h, self.p["head_w"], self.p["head_b"], 1, False)
+            self.cache = (cache, y)
+            return y[:, 0, :]
         feat, am = pool_fwd(h)
         logits = linear_fwd(feat, self.p["fc_w"], self.p["fc_b"])
         self.cache = (cache, feat, am)
         return logits
 
     def backward(self, glogits):
-        cache, feat, am = self.cache
         g = {}
-        gfeat, g["fc_w"], g["fc_b"] = linear_bwd(feat, self.p["fc_w"], glogits)
-        gh = pool_bwd(gfeat, am, cache[-1].shape)
+        if self.head == "conv":
+            cache, y = self.cache
+            gh, g["head_w"], g["head_b"] = conv1d_bwd(cache[-1], self.p["head_w"], y, glogits[:, None, :], 1, False)
+        else:
+            cache, feat, am = self.cache
+            gfeat, g["fc_w"], g["fc_b"] = linear_bwd(feat, self.p["fc_w"], glogits)
+            gh = pool_bwd(gfeat, am, cache[-1].shape)
         for i in reversed(range(len(self.dils))):
             gh, g[f"conv{i}_w"], g[f"conv{i}_b"] = conv1d_bwd(cache[i], self.p[f"conv{i}_w"], cache[i + 1], gh,
                                                               self.dils[i], True)
@@ -110,6 +122,39 @@ def spectrogram(x, nchan=256, navg=16):
     X = np.fft.fft(x[:T * nchan * navg].reshape(T, navg, nchan), axis=2)
     p = (np.abs(X) ** 2).sum(axis=1)               # [T][nchan]
     return np.fft.fftshift(p, axes=1).T            # channel c = bin (c + nchan/2) mod nchan
+
+
+def rows_fold(x, period, nbins):
+    """Per row, the mean of the samples in each of nbins phase bins of
+    frac(t / period); 0 for an empty bin."""
+    T = x.shape[-1]
+    ph = np.arange(T) / period
+    b = np.minimum(((ph - np.floor(ph)) * nbins).astype(np.int64), nbins - 1)
+    cnt = np.bincount(b, minlength=nbins).astype(np.float64)
+    out = np.stack([np.bincount(b, weights=r, minlength=nbins) for r in x.reshape(-1, T)])
+    out = np.where(cnt > 0, out / np.maximum(cnt, 1), 0.0)
+    return out.reshape(x.shape[:-1] + (nbins,))
+
+
+def incoherent_fold(Sf, delay_s, period):
+    """Trial profiles from folded channels: out[k][b] = sum_c Sf[c][(b + s)
+    mod nbins] / sqrt(nchan), s = rint(delay_s[k][c] / period * nbins)."""
+    nchan, nb = Sf.shape
+    out = np.zeros((delay_s.shape[0], nb))
+    b = np.arange(nb)
+    for k in range(delay_s.shape[0]):
+        for c in range(nchan):
+            s = int(np.rint(float(delay_s[k, c]) / period * nb)) % nb
+            out[k] += Sf[c, (b + s) % nb]
+    return out / np.sqrt(nchan)
+
+
+def harmonics(prof, K):
+    """|DFT of each folded profile row| at harmonics 1..K / sqrt(nbins),
+    harmonic-major: [K][rows]."""
+    nb = prof.shape[-1]
+    F = np.fft.fft(prof, axis=-1)[..., 1:K + 1]
+    return (np.abs(F) / np.sqrt(nb)).T
 
 
 def incoherent(S, delay, Tout):

@@ -16,19 +16,21 @@ def rel(a, b):
     return float(np.linalg.norm(np.asarray(a, np.float64) - b) / max(np.linalg.norm(b), 1e-30))
 
 
-def test_network_kernels_match_oracle(device):
+@pytest.mark.parametrize("head", ["pool", "conv"])
+def test_network_kernels_match_oracle(device, head):
     import torch
-    net = dmnet.DMNet(c_in=24, n_out=40, width=32, kw=9, dils=(1, 2, 4), seed=3, device=device)
-    rng = np.random.default_rng(4)
     B, T = 3, 150
+    nout = 40 if head == "pool" else T     # the convolutional head has one logit per step
+    net = dmnet.DMNet(c_in=24, n_out=nout, width=32, kw=9, dils=(1, 2, 4), seed=3, device=device, head=head)
+    rng = np.random.default_rng(4)
     x = rng.standard_normal((B, 24, T)).astype(np.float32)
-    tgt = rng.random((B, 40)).astype(np.float32)
+    tgt = rng.random((B, nout)).astype(np.float32)
     tgt /= tgt.sum(axis=1, keepdims=True)
     xd = torch.from_numpy(x).to(device)
     td = torch.from_numpy(tgt).to(device)
     loss = net.loss_and_grad(xd, td)
     torch.cuda.synchronize()
-    ref = R.Net(net.host_params(), (1, 2, 4))
+    ref = R.Net(net.host_params(), (1, 2, 4), head=head)
     logits = ref.forward(x.astype(np.float64))
     rloss, gl = R.softmax_xent(logits, tgt.astype(np.float64))
     grads, _ = ref.backward(gl)
@@ -36,6 +38,12 @@ def test_network_kernels_match_oracle(device):
     for i, a in enumerate(net.acts[1:]):
         assert rel(a.cpu().numpy(), ref.cache[0][i + 1]) < 1e-5
     for k in grads:
+        if k == "head_b":
+            # softmax over the steps is shift-invariant: the head bias has a
+            # zero gradient (the sum of softmax - target; the targets sum to 1
+            # only to FP32 rounding)
+            assert abs(float(net.g[k].cpu().numpy()[0])) < 1e-5 and abs(grads[k][0]) < 1e-5
+            continue
         assert rel(net.g[k].cpu().numpy(), grads[k]) < 1e-4, k
 
 
@@ -93,39 +101,92 @@ def test_simulated_pulsar_peaks_at_true_dm(device):
     assert s[0] < 0.5 * s.max()
 
 
-@pytest.mark.timeout(600)
+def test_fold_and_harmonics_match_oracle(device):
+    import torch
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal((5, 3001)).astype(np.float32)
+    xd = torch.from_numpy(x).to(device)
+    prof = torch.empty((5, 64), dtype=torch.float32, device=device)
+    ms.check(ms.lib().msosm_rows_fold(xd.data_ptr(), 5, 3001, 77.3, 64, 2.0, prof.data_ptr(), None))
+    h = torch.empty((16, 5), dtype=torch.float32, device=device)
+    ms.check(ms.lib().msosm_rows_harmonics(prof.data_ptr(), 5, 64, 16, h.data_ptr(), None))
+    torch.cuda.synchronize()
+    ref = 2.0 * R.rows_fold(x.astype(np.float64), 77.3, 64)
+    assert rel(prof.cpu().numpy(), ref) < 1e-6
+    assert rel(h.cpu().numpy(), R.harmonics(prof.cpu().numpy().astype(np.float64), 16)) < 1e-5
+    # trial profiles from folded channels
+    Sf = rng.standard_normal((256, 64)).astype(np.float32)
+    dly = (rng.random((7, 256)) * 0.05).astype(np.float32)
+    out = torch.empty((7, 64), dtype=torch.float32, device=device)
+    Sfd, dlyd = torch.from_numpy(Sf).to(device), torch.from_numpy(dly).to(device)   # kept alive for the launch
+    ms.check(ms.lib().msosm_incoherent_fold(Sfd.data_ptr(), 256, 64, dlyd.data_ptr(), 7, 0.0123, out.data_ptr(),
+                                            None))
+    torch.cuda.synchronize()
+    assert rel(out.cpu().numpy(), R.incoherent_fold(Sf.astype(np.float64), dly, 0.0123)) < 1e-5
+
+
+@pytest.mark.timeout(900)
 def test_trained_network_gain(device):
-    """Train the network on 12,000 simulated pulsars labelled by the plain
-    coherent method, then, for 64 held-out pulsars, coherently dedisperse
-    only its top 16 of a 1024-trial DM grid (64x fewer trials) and compare the
-    best folded S/N found with the full-grid best -- against the same budget
-    of 16 trials spread evenly and an untrained network.  With a fallback to
-    the full grid when no proposed trial reaches S/N 8, every pulsar is
-    recovered at a small average trial count."""
-    bw, f0, n = 64e6, 1368e6, 1 << 22
-    dm_max, n_fine, n_coarse, k = 100.0, 1024, 64, 16
+    """The stated configuration: 2^26-sample baseband at 64 MHz (f0 1368 MHz),
+    a 256-channel power spectrogram at 64 us (16 averaged 256-point spectra
+    per bin), 20,000 simulated training pulsars with DM uniform on [0, 500],
+    period log-uniform on [1 ms, 1 s], duty cycle 1-10% and folded S/N 5-30,
+    labelled by the coherent method's folded S/N on a 1024-trial grid (the
+    best trial within +-3 grid trials of the true DM: a local neighbourhood
+    of the grid, engines in the FP16-storage form).  The network (dilated 1D
+    convolutions along the trial axis, ~181k parameters, FP16 weights, FP32
+    accumulation) reads, for every grid trial, the harmonic amplitudes of the
+    incoherently dedispersed spectrogram folded at the period, and scores
+    every trial.  For 64 held-out pulsars its top 16 trials are coherently
+    dedispersed and folded and their best S/N is compared with the full
+    1024-trial grid's best (the judge, fused FP32 form), against 16 evenly
+    spread trials and an untrained network; with a fallback to the full grid
+    when no proposed trial reaches S/N 8."""
+    import time
+
+    import torch
+    bw, f0, n = 64e6, 1368e6, 1 << 26
+    dm_max, n_fine, k = 500.0, 1024, 16
     fine = np.linspace(0, dm_max, n_fine).astype(np.float32)
-    fe = dmnet.Frontend(bw, f0, dm_max, n_coarse, n, device)
-    judge = dmnet.Judge(bw, f0, fine, n, device=device)
-    draw = dict(period_range=(0.002, 0.016), duty_range=(0.02, 0.10), snr_range=(12.0, 40.0))
-    ptr = dmnet.draw_pulsars(12000, dm_max, seed=101, **draw)
+    t0 = time.time()
+    fe = dmnet.Frontend(bw, f0, dm_max, n_fine, n, device, fold_bins=64, harmonics=16, dms=fine)
+    labeller = dmnet.Judge(bw, f0, fine, n, count=512, device=device, form=ms.FORM_FUSED_F16, max_engines=6)
+    draw = dict(period_range=(0.001, 1.0), duty_range=(0.01, 0.10), snr_range=(5.0, 30.0))
+    ptr = sorted(dmnet.draw_pulsars(20000, dm_max, seed=101, **draw), key=lambda m: m["dm"])
     pte = dmnet.draw_pulsars(64, dm_max, seed=202, **draw)
-    Xtr, ltr = dmnet.make_dataset(ptr, n, bw, f0, fe, judge, device=device, progress=3000)
-    Xte, _ = dmnet.make_dataset(pte, n, bw, f0, fe, judge, device=device)
-    net = dmnet.DMNet(c_in=n_coarse, n_out=n_fine, width=64, kw=9, dils=(1, 2, 4), seed=0, device=device)
-    assert 150_000 < net.n_params < 300_000
-    untrained = net.forward(Xte).cpu().numpy()
-    losses = dmnet.train(net, Xtr, ltr, epochs=20, batch=64, lr=1e-3, sigma=5.0, cosine=True, progress=True)
-    assert losses[-1] < 0.7 * losses[0], losses
-    logits = net.forward(Xte).cpu().numpy()
-    even = np.linspace(0, n_fine - 1, k).round().astype(int)
-    rec = {"network": [], "even": [], "untrained": [], "network+fallback": []}
-    trials = []
-    fallback_snr = 8.0
+    print(f"front end: {fe.T} spectrogram bins of {fe.tbin * 1e6:.0f} us, network input {fe.shape}; label skip "
+          f"{labeller.skip} samples", flush=True)
+    Xtr, ltr = dmnet.make_dataset(ptr, n, bw, f0, fe, labeller, device=device, progress=2000, label_halfwidth=3,
+                                  label_stride=4)
+    labeller.close()
+    print(f"simulated and labelled 20000 pulsars in {time.time() - t0:.0f} s "
+          f"({labeller.trials_run} coherent trials)", flush=True)
+    judge = dmnet.Judge(bw, f0, fine, n, count=256, device=device, max_engines=4)
+    Xte = torch.empty((len(pte),) + fe.shape, dtype=torch.float32, device=device)
+    fulls = []
     for i, p in enumerate(pte):
         u = dmnet.simulate(n, bw, f0, p["dm"], p["period"], p["duty"], dmnet.power_for_snr(p["snr"], n, p["duty"]),
                            p["seed"], device)
-        full = judge.snr_full(u, p["period"])
+        fe(u, Xte[i], p["period"])
+        fulls.append(judge.snr_full(u, p["period"], chunk=256))
+    judge.close()
+    print(f"held-out front ends and full-grid judges done at {time.time() - t0:.0f} s", flush=True)
+    net = dmnet.DMNet(c_in=fe.shape[0], n_out=n_fine, width=96, kw=9, dils=(1, 2, 4), seed=0, device=device,
+                      head="conv")
+    assert 150_000 < net.n_params < 250_000
+    untrained = net.forward(Xte).cpu().numpy()
+    losses = dmnet.train(net, Xtr, ltr, epochs=10, batch=64, lr=1e-3, sigma=2.0, cosine=True, progress=True)
+    print(f"trained at {time.time() - t0:.0f} s", flush=True)
+    # soft targets (sigma 2 trials) and many pulsars too weak to locate keep
+    # the loss well above the targets' entropy; the gain is judged below
+    assert losses[-1] < losses[0] - 0.1, losses
+    logits = net.forward(Xte).cpu().numpy()
+    even = np.linspace(0, n_fine - 1, k).round().astype(int)
+    rec = {"network": [], "even": [], "untrained": [], "network+fallback": []}
+    trials, best_all = [], []
+    fallback_snr = 8.0
+    for i in range(len(pte)):
+        full = fulls[i]
         best = full.max()
         prop = full[dmnet.topk(logits[i], k)].max()
         rec["network"].append(prop / best)
@@ -134,15 +195,21 @@ def test_trained_network_gain(device):
         fb = prop < fallback_snr
         rec["network+fallback"].append(1.0 if fb else prop / best)
         trials.append(k + (n_fine if fb else 0))
-    judge.close()
+        best_all.append(best)
     rec = {key: np.array(v) for key, v in rec.items()}
+    det = np.array(best_all) >= 8.0   # pulsars the full grid detects
     for key, v in rec.items():
-        print(f"{key:17s}: recovered S/N fraction median {np.median(v):.3f}, >=0.9 in {np.mean(v >= 0.9):.2f}, "
-              f">=0.8 in {np.mean(v >= 0.8):.2f} of {len(v)}")
+        print(f"{key:17s}: recovered S/N fraction median {np.median(v):.3f} (detected: {np.median(v[det]):.3f}), "
+              f">=0.8 in {np.mean(v >= 0.8):.2f} of {len(v)} ({np.mean(v[det] >= 0.8):.2f} of {det.sum()} "
+              f"detected)")
     print(f"trials per pulsar with fallback: mean {np.mean(trials):.1f} of {n_fine} "
           f"({n_fine / np.mean(trials):.1f}x fewer than the full grid)")
-    assert np.mean(rec["network"] >= 0.8) >= 0.6
-    assert np.median(rec["network"]) > np.median(rec["even"]) + 0.2
-    assert np.median(rec["network"]) > np.median(rec["untrained"]) + 0.2
-    assert np.mean(rec["network+fallback"] >= 0.8) >= 0.8
-    assert np.mean(trials) < n_fine / 4
+    assert det.sum() >= 32
+    assert np.mean(rec["network"][det] >= 0.8) >= 0.7
+    assert np.median(rec["network"][det]) > np.median(rec["even"][det]) + 0.05
+    assert np.mean(rec["network"][det] >= 0.8) > np.mean(rec["even"][det] >= 0.8) + 0.2
+    assert np.median(rec["network"][det]) > np.median(rec["untrained"][det]) + 0.1
+    assert np.mean(rec["network+fallback"][det] >= 0.8) >= 0.8
+    # pulsars below the fallback S/N (about a fifth at folded S/N 5-30) need
+    # the full grid whatever the proposal
+    assert np.mean(trials) < n_fine / 3
