@@ -299,7 +299,7 @@ def osm_rows(ms, bw, dm, f0, d_in, forms):
 def other_configs(ms, torch, dev):
     """Short runs of the other workloads (inputs resident in HBM, uniform
     random codes): a single trial at 16 MHz (DM 10, FFT 1024) and at 128 MHz
-    (DM 100, FFT 2^16, which only the reference-structured form takes), and
+    (DM 100 at its plan, FFT 2^16 with K 152, and at FFT 2^14 with K 608), and
     24 independent streams of 8-64 MHz, 2^20-2^25 samples, 1-64 trials,
     autotuned FFT sizes, up to 256 blocks per call, each phase-folded,
     replayed as one graph."""
@@ -320,6 +320,9 @@ def other_configs(ms, torch, dev):
                                                                           count=256, form=ms.FORM_PLAIN_F32),
                                                      d_in, L, steps=2, gate=1e-5)}
     for name, form, gate in (("mixed_f16", ms.FORM_FUSED_F16, 1e-3), ("fused_f32", ms.FORM_FUSED_F32, 1e-5)):
+        c2[name + ", FFT 2^16 (K 152)"] = time_engine(ms, torch, ms.Plan(128e6, 100.0, 1336e6, fftpoint=65536,
+                                                                         count=256, form=form), d_in, L, steps=2,
+                                                      gate=gate)
         c2[name + ", FFT 2^14 (K 608)"] = time_engine(ms, torch, ms.Plan(128e6, 100.0, 1336e6, fftpoint=16384,
                                                                          count=1024, form=form), d_in, L, steps=2,
                                                       gate=gate)

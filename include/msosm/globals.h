@@ -74,10 +74,10 @@ struct Core {
         check(msosm_plan_create(&d, &plan), "plan");
         check(msosm_plan_get_info(plan, &info), "plan info");
         int rc = MSOSM_ERR_UNSUPPORTED;
-        // the fused kernels hold one transform in LDS (FFT sizes up to 16384),
-        // keep delays in 16 bits and address their spectra ring with 32-bit
-        // offsets; plans beyond that use the reference-structured form
-        if (d.form == MSOSM_FORM_PLAIN_F32 || info.fftpoint <= 16384)
+        // the fused kernels take FFT sizes up to 2^21, keep delays in 16 bits
+        // and address their spectra ring with 32-bit offsets; plans beyond
+        // that use the reference-structured form
+        if (d.form == MSOSM_FORM_PLAIN_F32 || info.fftpoint <= 2097152)
             rc = msosm_engine_create(plan, nullptr, &eng);
         if (rc == MSOSM_ERR_UNSUPPORTED && d.form != MSOSM_FORM_PLAIN_F32) {
             msosm_plan_destroy(plan);

@@ -170,10 +170,15 @@ int msosm_range_pop(void);
  *   hankel_tiles_per_group: 16-trial tiles that share one Hankel GEMM
  *     workgroup's staged delay band; rule: the largest whose band leaves LDS
  *     for three workgroups per CU (reduced until the band fits the LDS).
+ *   scratch_chunk_kib: fused forms above 16384 points run their forward
+ *     transform over a chunk of windows, and their inverse over a chunk of
+ *     (trial, block) units, at a time through an FP32 scratch of about this
+ *     size (KiB); rule and upper limit: 128 MiB.
  * get_tuning reports the requests, with the Hankel group size in effect. */
 typedef struct msosm_tuning {
     int inv_blocks_per_wg;
     int hankel_tiles_per_group;
+    int scratch_chunk_kib;
 } msosm_tuning;
 int msosm_engine_set_tuning(msosm_engine *eng, const msosm_tuning *tuning);
 int msosm_engine_get_tuning(msosm_engine *eng, msosm_tuning *tuning);

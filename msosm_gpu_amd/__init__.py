@@ -67,6 +67,7 @@ class Tuning(ctypes.Structure):
     _fields_ = [
         ("inv_blocks_per_wg", ctypes.c_int),
         ("hankel_tiles_per_group", ctypes.c_int),
+        ("scratch_chunk_kib", ctypes.c_int),
     ]
 
 
@@ -349,14 +350,15 @@ class Engine:
         nbytes = image.nbytes if isinstance(image, np.ndarray) else image.numel() * image.element_size()
         check(lib().msosm_engine_state_import(self.h, _ptr(image), nbytes, 1 if _is_device(image) else 0))
 
-    def set_tuning(self, inv_blocks_per_wg=0, hankel_tiles_per_group=0):
-        t = Tuning(inv_blocks_per_wg, hankel_tiles_per_group)
+    def set_tuning(self, inv_blocks_per_wg=0, hankel_tiles_per_group=0, scratch_chunk_kib=0):
+        t = Tuning(inv_blocks_per_wg, hankel_tiles_per_group, scratch_chunk_kib)
         check(lib().msosm_engine_set_tuning(self.h, ctypes.byref(t)))
 
     def get_tuning(self):
         t = Tuning()
         check(lib().msosm_engine_get_tuning(self.h, ctypes.byref(t)))
-        return dict(inv_blocks_per_wg=int(t.inv_blocks_per_wg), hankel_tiles_per_group=int(t.hankel_tiles_per_group))
+        return dict(inv_blocks_per_wg=int(t.inv_blocks_per_wg), hankel_tiles_per_group=int(t.hankel_tiles_per_group),
+                    scratch_chunk_kib=int(t.scratch_chunk_kib))
 
     def set_timing(self, on=True):
         check(lib().msosm_engine_set_timing(self.h, 1 if on else 0))

@@ -24,7 +24,7 @@ LIB = os.path.join(LIBDIR, "libmsosm.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("MSOSM_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["kernels.hip", "invmfma.hip", "hankel.hip", "engine.hip", "fold.hip", "batch.hip", "learn.hip", "outring.hip", "refsim.hip", "noalloc.hip"]
+HIP_SOURCES = ["kernels.hip", "invmfma.hip", "hankel.hip", "engine.hip", "fold.hip", "batch.hip", "learn.hip", "outring.hip", "refsim.hip", "noalloc.hip", "bigfft.hip"]
 HOST_SOURCES = ["plan.cpp"]
 
 HIP_FLAGS = [

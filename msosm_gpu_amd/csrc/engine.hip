@@ -14,6 +14,7 @@
 // the last M points of IDFT(X_{g-d(n)}[n] H[n]) / (2M)^2.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -112,6 +113,13 @@ struct msosm_engine {
     uint32_t *tail = nullptr;       // 2 x M: previous call's last M input samples / this call's
     void *xring = nullptr;          // rx*2M spectra (float2 or half2)
     int rx = 0;
+    // FUSED at 2^15..2^21 points (four-step): ring, H and delays in position
+    // order, FP32 scratch of the forward and of one inverse chunk
+    bool big = false;
+    float2 *Hbig = nullptr;         // [ndm][2M] H in position order (FUSED_F32)
+    float2 *big_fwd = nullptr;      // big_fwd_max*2M
+    float2 *big_inv = nullptr;      // big_inv_max*2M
+    int big_fwd_max = 0, big_inv_max = 0; // windows / units the scratch holds
     // HANKEL (delay-line accumulate as per-bin GEMMs on MFMA)
     void *ybuf = nullptr;           // [ndm][count][2M] products (half2 or float2)
     int hankel_mode = -1;
@@ -218,7 +226,8 @@ void free_engine(msosm_engine *e)
             (void)hipEventDestroy(ev);
     void *bufs[] = {e->in_stage, e->H32,  e->H16,     e->delay32,  e->delay16, e->delay_pk, e->Hp,
                     e->dpk_mfma, e->mfma_tab, e->out, e->out_u16, e->inbuf,
-                    e->fftblk, e->ring, e->ifftblk, e->fft_scratch, e->tail, e->xring, e->ybuf, e->hankel_lo};
+                    e->fftblk, e->ring, e->ifftblk, e->fft_scratch, e->tail, e->xring, e->ybuf, e->hankel_lo,
+                    e->Hbig, e->big_fwd, e->big_inv};
     for (void *b : bufs)
         if (b)
             (void)hipFree(b);
@@ -353,9 +362,11 @@ int create_engine(const msosm::Plan &plan, void *stream, msosm_engine **out)
                               "mixed-precision gate; use MSOSM_FORM_HANKEL_BF16X3 (hi+lo split) or "
                               "MSOSM_FORM_HANKEL_F16"));
     e->plan_hash = plan_identity(plan);
-    if (e->form != MSOSM_FORM_PLAIN_F32 && !msosm::fused_supported(e->log2n))
-        return bail(set_error(MSOSM_ERR_UNSUPPORTED, "fused forms support FFT sizes 4..16384 (got %lu); use "
-                                                     "MSOSM_FORM_PLAIN_F32", N));
+    e->big = (e->form == MSOSM_FORM_FUSED_F32 || e->form == MSOSM_FORM_FUSED_F16) && msosm::big_supported(e->log2n);
+    if (e->form != MSOSM_FORM_PLAIN_F32 && !msosm::fused_supported(e->log2n) && !e->big)
+        return bail(set_error(MSOSM_ERR_UNSUPPORTED, "the fused forms support FFT sizes 4..2097152, the matrix-core "
+                                                     "and Hankel forms 4..16384 (got %lu); use MSOSM_FORM_PLAIN_F32",
+                              N));
     if (e->form == MSOSM_FORM_FUSED_F16_MFMA && e->log2n != msosm::mfft::kLog2N)
         return bail(set_error(MSOSM_ERR_UNSUPPORTED, "MSOSM_FORM_FUSED_F16_MFMA needs an FFT size of %d (plan has %lu)",
                               msosm::mfft::kN, N));
@@ -449,6 +460,14 @@ int create_engine(const msosm::Plan &plan, void *stream, msosm_engine **out)
         }
         ALLOC(e->delay16, (size_t)ndm * N);
         ALLOC(e->tail, 2 * (size_t)M); // two halves, alternating by call
+        if (e->big) {
+            e->big_fwd_max = msosm::big_chunk(e->log2n, count, 0);
+            e->big_inv_max = msosm::big_chunk(e->log2n, (long long)ndm * count, 0);
+            ALLOC(e->big_fwd, (size_t)e->big_fwd_max * N);
+            ALLOC(e->big_inv, (size_t)e->big_inv_max * N);
+            if (!f16)
+                ALLOC(e->Hbig, (size_t)ndm * N);
+        }
         if (f16) {
             ALLOC(e->H16, (size_t)ndm * N);
             __half2 *r = nullptr;
@@ -464,7 +483,16 @@ int create_engine(const msosm::Plan &plan, void *stream, msosm_engine **out)
         HIP_TRY(hipMemsetAsync(e->tail, 0, sizeof(uint32_t) * 2 * M, e->stream));
         int32_t *tmp = nullptr;
         ALLOC(tmp, (size_t)ndm * N);
-        herr = hipMemcpyAsync(tmp, p.delay.data(), sizeof(int32_t) * ndm * N, hipMemcpyHostToDevice, e->stream);
+        std::vector<int32_t> perm;
+        const int32_t *dsrc = p.delay.data();
+        if (e->big) { // delays in the ring's position order
+            perm.resize((size_t)ndm * N);
+            for (int d = 0; d < ndm; d++)
+                for (size_t q = 0; q < N; q++)
+                    perm[(size_t)d * N + q] = p.delay[(size_t)d * N + (size_t)msosm::big_position_bin(e->log2n, (long long)q)];
+            dsrc = perm.data();
+        }
+        herr = hipMemcpyAsync(tmp, dsrc, sizeof(int32_t) * ndm * N, hipMemcpyHostToDevice, e->stream);
         if (herr == hipSuccess)
             herr = msosm::launch_i32_to_u16(e->delay16, tmp, (size_t)ndm * N, e->stream);
         (void)hipStreamSynchronize(e->stream);
@@ -510,6 +538,9 @@ int create_engine(const msosm::Plan &plan, void *stream, msosm_engine **out)
     int rc = build_filter_tables(e);
     if (rc != MSOSM_OK)
         return bail(rc);
+    if (e->big) // H/(2M) (FP16) or H (FP32) in the ring's position order
+        HIP_TRY(msosm::launch_big_permute(e->H32, e->H16 ? (void *)e->H16 : (void *)e->Hbig, e->H16 != nullptr,
+                                          e->H16 ? 0.5f / (float)M : 1.0f, e->log2n, ndm, e->stream));
     if (e->form == MSOSM_FORM_FUSED_F16_MFMA) {
         HIP_TRY(msosm::launch_inv_mfma_filter(e->Hp, e->H16, ndm, e->stream));
         HIP_TRY(msosm::launch_inv_mfma_tables(e->mfma_tab, e->stream));
@@ -605,6 +636,25 @@ int filter_fused(msosm_engine *e, const uint32_t *cur)
     const bool mfma = e->form == MSOSM_FORM_FUSED_F16_MFMA;
     const bool f16 = e->form == MSOSM_FORM_FUSED_F16 || mfma;
     hipStream_t s = e->stream;
+    if (e->big) {
+        // scratch chunks: the requested size, within what was allocated
+        const size_t req = (size_t)e->tuning.scratch_chunk_kib << 10;
+        const int fw = std::min(e->big_fwd_max, msosm::big_chunk(e->log2n, count, req));
+        const int iu = std::min(e->big_inv_max, msosm::big_chunk(e->log2n, (long long)ndm * count, req));
+        {
+            StageScope st(e, MSOSM_STAGE_FFT_FWD);
+            HIP_TRY(msosm::launch_big_forward(cur, tail_in(e), tail_out(e), e->xring, e->big_fwd, fw, f16, e->log2n,
+                                              count, e->g0, e->rx, s));
+        }
+        {
+            StageScope st(e, MSOSM_STAGE_FFT_INV);
+            HIP_TRY(msosm::launch_big_inverse(e->xring, f16 ? (const void *)e->H16 : (const void *)e->Hbig,
+                                              e->delay16, e->big_inv, iu, e->out, f16, e->log2n, count, ndm, e->g0,
+                                              e->rx, s));
+        }
+        e->g0 += count;
+        return MSOSM_OK;
+    }
     {
         StageScope st(e, MSOSM_STAGE_FFT_FWD);
         HIP_TRY(msosm::launch_fwd_u16(cur, tail_in(e), tail_out(e), e->xring, f16, e->log2n, count, e->g0, e->rx,
@@ -1007,9 +1057,10 @@ int msosm_engine_state_import(msosm_engine *e, const void *src, size_t bytes, in
 
 int msosm_engine_set_tuning(msosm_engine *e, const msosm_tuning *t)
 {
-    if (!e || !t || t->inv_blocks_per_wg < 0 || t->inv_blocks_per_wg > 64 || t->hankel_tiles_per_group < 0)
+    if (!e || !t || t->inv_blocks_per_wg < 0 || t->inv_blocks_per_wg > 64 || t->hankel_tiles_per_group < 0 ||
+        t->scratch_chunk_kib < 0)
         return set_error(MSOSM_ERR_ARG, "set_tuning: bad argument (0 <= inv_blocks_per_wg <= 64, "
-                                        "hankel_tiles_per_group >= 0)");
+                                        "hankel_tiles_per_group >= 0, scratch_chunk_kib >= 0)");
     if (form_hankel(e->form) && t->hankel_tiles_per_group != e->tuning.hankel_tiles_per_group) {
         int tpg = 0, kspan = 0;
         std::vector<int> lo;

@@ -41,6 +41,28 @@ int fused_delay_threads(int log2n);
 bool fused_uses_packed_delay(int log2n);
 bool fused_supported(int log2n);
 
+// Shared-ring forms at 2^15 .. 2^21 points (bigfft.hip): four-step transforms
+// with rows of 2^14 points.  The spectra ring, the filter table and the delay
+// table hold bin big_position_bin(p) at position p.  launch_big_permute
+// writes rows of natural-order complex64 src into position order (FP16 or
+// FP32, times scale).  Forward: count windows, chunk at a time through a
+// scratch of chunk << log2n float2, into ring slots (g0 + w) mod rx;
+// inverse: all (trial, block) units, chunk at a time through a scratch of
+// chunk << log2n float2; output [dm][count][M] as the other fused forms.
+// big_chunk: how many of `items` windows or units fit `bytes` of scratch
+// (0: the default of 128 MiB), at least 1.
+bool big_supported(int log2n);
+long long big_position_bin(int log2n, long long p);
+int big_chunk(int log2n, long long items, size_t bytes);
+hipError_t launch_big_permute(const float2 *src, void *dst, bool f16, float scale, int log2n, long long rows,
+                              hipStream_t s);
+hipError_t launch_big_forward(const uint32_t *cur, const uint32_t *tail, uint32_t *tail_out, void *ring,
+                              float2 *scratch, int chunk, bool f16, int log2n, int count, long long g0, int rx,
+                              hipStream_t s);
+hipError_t launch_big_inverse(const void *ring, const void *Hp, const uint16_t *dp, float2 *scratch, int chunk,
+                              float2 *out, bool f16, int log2n, int count, int ndm, long long g0, int rx,
+                              hipStream_t s);
+
 // Fused inverse with the DFT stages on the matrix cores (invmfma.hip), 2^14
 // points: ring, Hp and dpk in the gather order of mfft::ring_pos (Hp from
 // launch_inv_mfma_filter; dpk[dm][pp] = d(p0) | d(p0 + 64) << 16 with

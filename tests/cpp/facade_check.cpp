@@ -3,7 +3,10 @@
 // multiple_dm_concurrent_msosm.cpp, msosm_psrdada.cpp) and writes every
 // output to files that tests/test_gpu_facade.py checks against the oracle.
 //
-// usage: facade_check <input.u16> <num_samples> <out_prefix>
+// usage: facade_check <input.u16> <num_samples> <out_prefix> [default_plan]
+//   default_plan: only the single-DM engine at the reference recording
+//   program's plan (fftpoint 2097152, count 1, msosm_psrdada.cpp:94-95) at
+//   128 MHz, DM 100, f0 1336 MHz
 #include <cstdio>
 #include <fstream>
 #include <vector>
@@ -38,6 +41,21 @@ int main(int argc, char **argv)
             std::fprintf(stderr, "cannot read %lu samples from %s\n", L, argv[1]);
             return 2;
         }
+    }
+    if (argc >= 5 && std::string(argv[4]) == "default_plan") {
+        std::remove((pre + ".big.c64").c_str());
+        MSOSM_GPU_BATCH big(128e6f, 100.0f, 1336e6f);
+        big.initialize_uint16(2097152, 1);
+        std::ofstream meta(pre + ".meta");
+        meta << "big M " << big.M << " Nd " << big.Nd << " K " << big.delaycount << " count " << big.count << "\n";
+        std::vector<Complex> out((size_t)big.count * big.M);
+        const unsigned long per = (unsigned long)big.count * big.M;
+        for (unsigned long c = 0; c + per <= L; c += per) {
+            big.filter_block_uint16(in.data() + c);
+            big.get_output(out.data());
+            dump(pre + ".big.c64", out.data(), out.size());
+        }
+        return 0;
     }
     for (const char *s : {".batch.c64", ".conc.c64", ".conc.u16", ".osm.c64", ".loop.c64", ".fold.f32", ".inten.f32",
                           ".dmstream.c64", ".dmstream.u16", ".bstream0.c64", ".bstream1.c64", ".sim.u16", ".sim.c64"})

@@ -85,3 +85,32 @@ def test_facade_program(device, tmp_path):
     sig, u16, _ = ms.refsim_generate(bw, 10.0, f0, 0.005, repeat=4)
     np.testing.assert_array_equal(np.fromfile(pre + ".sim.u16", np.uint16), u16)
     np.testing.assert_array_equal(np.fromfile(pre + ".sim.c64", np.float32), sig.view(np.float32))
+
+
+def test_facade_reference_default_plan(device, tmp_path):
+    """MSOSM_GPU_BATCH at the recording program's plan (fftpoint 2^21, count
+    1, msosm_psrdada.cpp:94-95): 128 MHz, DM 100 (K 5), eight calls, every
+    output block against the FP64 same-plan oracle.  The facade runs the
+    fused FP32 form, now four-step at this size."""
+    if not os.path.exists(BIN):
+        pytest.fail("tests/cpp/facade_check is not built (run __graft_entry__.build())")
+    rng = np.random.Generator(np.random.Philox(78))
+    M = 1 << 20
+    L = 8 * M
+    u = np.clip(np.round(32768 + 900 * rng.standard_normal(2 * L)), 1, 65535).astype(np.uint16)
+    inp = tmp_path / "in.u16"
+    u.tofile(inp)
+    pre = str(tmp_path / "out")
+    r = subprocess.run([BIN, str(inp), str(L), pre, "default_plan"], capture_output=True, text=True, timeout=240)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0, r.stderr
+    meta = open(pre + ".meta").read()
+    assert "big M 1048576 Nd 4976580 K 5 count 1" in meta, meta
+    y = np.fromfile(pre + ".big.c64", np.complex64)
+    assert y.size == L
+    bw, dm, f0 = 128e6, 100.0, 1336e6
+    x = oracle.unpack_u16(u).astype(np.complex128)
+    ref = oracle.msosm_run(x, M, oracle.delay_points(bw, dm, f0, M)[0], oracle.dedisp_params(bw, dm, f0, M))
+    err = rel_l2(y, ref)
+    print(f"facade at FFT 2^21, count 1: rel L2 vs FP64 {err:.2e}")
+    assert err < 1e-5
