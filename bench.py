@@ -301,8 +301,8 @@ def other_configs(ms, torch, dev):
     random codes): a single trial at 16 MHz (DM 10, FFT 1024) and at 128 MHz
     (DM 100 at its plan, FFT 2^16 with K 152, and at FFT 2^14 with K 608), and
     24 independent streams of 8-64 MHz, 2^20-2^25 samples, 1-64 trials,
-    autotuned FFT sizes, up to 256 blocks per call, each phase-folded,
-    replayed as one graph."""
+    autotuned FFT sizes, forms and launch shapes, up to 2^22 samples per call,
+    each phase-folded, replayed as one graph."""
     out = {}
     L = 1 << 22
     d_in = random_u16_words(torch, dev, L, 0x5EED0001)
@@ -331,7 +331,7 @@ def other_configs(ms, torch, dev):
     del d_in
     torch.cuda.empty_cache()
     rng = np.random.Generator(np.random.Philox(0x5EED0004))
-    engines, inputs, lens, periods, work = [], [], [], [], 0
+    engines, inputs, lens, periods, shapes, work = [], [], [], [], [], 0
     for s in range(24):
         bw = float(rng.choice([8e6, 16e6, 32e6, 64e6]))
         f0 = 1400e6 - bw / 2
@@ -339,14 +339,20 @@ def other_configs(ms, torch, dev):
         ndm = int(rng.integers(1, 65))
         dms = np.sort(rng.uniform(0, 300, ndm)).astype(np.float32)
         period = float(np.exp(rng.uniform(np.log(1e-3), np.log(1.0))))
-        fp, _, form = ms.autotune_fftpoint(bw, dms, f0, min_fftpoint=1024, max_fftpoint=65536,
-                                           target_samples=min(Ls, 1 << 19), calls=1, form=ms.FORM_FUSED_F16)
+        tuned = ms.autotune(bw, dms, f0, min_fftpoint=1024, max_fftpoint=65536, target_samples=min(Ls, 1 << 22),
+                            calls=1, form=ms.FORM_FUSED_F16)
+        fp, form = tuned["fftpoint"], tuned["form"]
         M = fp // 2
-        count = max(1, min(256, Ls // M))
+        # up to 2^22 samples per call: small transforms get many blocks per
+        # launch instead of many launches
+        count = max(1, min(Ls, 1 << 22) // M)
         plan = ms.Plan(bw, dms, f0, fftpoint=fp, count=count, form=form)
         n = plan.count * plan.M
         used = (Ls // n) * n
         engines.append(ms.Engine(plan))
+        engines[-1].set_tuning(**tuned["tuning"])
+        shapes.append(dict(bw_mhz=bw / 1e6, log2_samples=int(np.log2(Ls)), trials=ndm, fftpoint=fp, form=form,
+                           count=count))
         inputs.append(random_u16_words(torch, dev, used, 1000 + s))
         lens.append(used)
         periods.append(period * bw)
@@ -361,10 +367,26 @@ def other_configs(ms, torch, dev):
         batch.launch()
     batch.synchronize()
     dt = (time.perf_counter() - t0) / reps
-    out["24 streams, 8-64 MHz, up to 256 blocks per call, one graph with phase folds"] = {
-        "value": round(work / dt / 1e6, 1), "ms_per_pass": round(dt * 1e3, 3), "dm_trial_samples": work,
-        "graph_nodes": batch.graph_nodes}
     batch.close()
+    # each stream alone, as a graph of its own: where the pass's time goes
+    per = []
+    for k in range(len(engines)):
+        b1 = ms.Batch([engines[k]], [inputs[k]], [lens[k]], [periods[k]], nbins=512)
+        b1.launch()
+        b1.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(reps):
+            b1.launch()
+        b1.synchronize()
+        ms1 = (time.perf_counter() - t1) / reps * 1e3
+        b1.close()
+        per.append(dict(shapes[k], ms=round(ms1, 3),
+                        value=round(engines[k].plan.num_dms * lens[k] / (ms1 / 1e3) / 1e6, 1)))
+    per.sort(key=lambda r: -r["ms"])
+    out["24 streams, 8-64 MHz, up to 2^22 samples per call, one graph with phase folds"] = {
+        "value": round(work / dt / 1e6, 1), "ms_per_pass": round(dt * 1e3, 3), "dm_trial_samples": work,
+        "graph_nodes": batch.graph_nodes, "streams_alone_ms_sum": round(sum(r["ms"] for r in per), 3),
+        "per_stream_alone": per}
     for e in engines:
         e.close()
     return out

@@ -315,13 +315,27 @@ int msosm_batch_info(msosm_batch *b, int *num_streams, size_t *graph_nodes);
 int msosm_batch_profiles(msosm_batch *b, int i, float *prof, float *hits);
 int msosm_batch_destroy(msosm_batch *b);
 
-/* Autotuner (no reference counterpart): among powers of two in
- * [min_fftpoint, max_fftpoint] that are not below the plan's minimum, time
- * `calls` filter calls of about target_samples samples on synthetic input and
- * return the FFT size with the highest DM-trial*sample throughput (results are
- * cached per descriptor for the life of the process).  Sizes above 2^14 are
- * timed with MSOSM_FORM_PLAIN_F32 when desc->form is a single-workgroup form;
- * *best_form (may be NULL) receives the form the chosen size runs with. */
+/* Autotuner (no reference counterpart; the reference's callers pick a
+ * per-stream fftpoint by hand, msosm_stream/msosm_gpu_batch_stream.cpp:33-103).
+ * Among powers of two in [min_fftpoint, max_fftpoint] that are not below the
+ * plan's minimum, and for each size among the launch shapes its form offers
+ * (msosm_tuning: inverse blocks per workgroup at 4096..16384 points, Hankel
+ * tiles per group, scratch chunk above 16384 points), time `calls` filter
+ * calls of about target_samples samples on synthetic input and return the
+ * fastest in DM-trial*samples per second (cached per descriptor for the life
+ * of the process).  The form is desc->form where it runs at that size; the
+ * matrix-core and Hankel forms (16384 points at most) run as FUSED_F16 above,
+ * the fused forms as PLAIN_F32 above 2^21. */
+typedef struct msosm_autotune_result {
+    unsigned long fftpoint;
+    int form;                /* the form the chosen size runs with           */
+    msosm_tuning tuning;     /* launch shapes to give msosm_engine_set_tuning */
+    double ns_per_dm_sample;
+    int candidates;          /* configurations timed                         */
+} msosm_autotune_result;
+int msosm_autotune(const msosm_plan_desc *desc, unsigned long min_fftpoint, unsigned long max_fftpoint,
+                   unsigned long target_samples, int calls, void *stream, msosm_autotune_result *result);
+/* The same, returning only the size, its time and its form. */
 int msosm_autotune_fftpoint(const msosm_plan_desc *desc, unsigned long min_fftpoint, unsigned long max_fftpoint,
                             unsigned long target_samples, int calls, void *stream, unsigned long *best_fftpoint,
                             double *best_ns_per_dm_sample, int *best_form);

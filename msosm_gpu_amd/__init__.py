@@ -71,6 +71,16 @@ class Tuning(ctypes.Structure):
     ]
 
 
+class AutotuneResult(ctypes.Structure):
+    _fields_ = [
+        ("fftpoint", ctypes.c_ulong),
+        ("form", ctypes.c_int),
+        ("tuning", Tuning),
+        ("ns_per_dm_sample", ctypes.c_double),
+        ("candidates", ctypes.c_int),
+    ]
+
+
 # Exported C ABI: name -> (restype, argtypes)
 _vp, _i, _ul, _sz = ctypes.c_void_p, ctypes.c_int, ctypes.c_ulong, ctypes.c_size_t
 _f, _ll = ctypes.c_float, ctypes.c_longlong
@@ -143,6 +153,7 @@ SIGNATURES = {
     "msosm_batch_info": (_i, [_vp, ctypes.POINTER(_i), ctypes.POINTER(_sz)]),
     "msosm_batch_profiles": (_i, [_vp, _i, _vp, _vp]),
     "msosm_batch_destroy": (_i, [_vp]),
+    "msosm_autotune": (_i, [ctypes.POINTER(PlanDesc), _ul, _ul, _ul, _i, _vp, ctypes.POINTER(AutotuneResult)]),
     "msosm_autotune_fftpoint": (_i, [ctypes.POINTER(PlanDesc), _ul, _ul, _ul, _i, _vp, ctypes.POINTER(_ul),
                                      ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_i)]),
     "msosm_sim_pulsar": (_i, [_vp, _sz, ctypes.c_double, ctypes.c_double, _f, ctypes.c_uint, _vp]),
@@ -611,6 +622,27 @@ def autotune_fftpoint(bw, dm, f0, min_fftpoint=1024, max_fftpoint=65536, target_
                                         _stream_handle(stream), ctypes.byref(best), ctypes.byref(ns),
                                         ctypes.byref(bf)))
     return int(best.value), float(ns.value), int(bf.value)
+
+
+def autotune(bw, dm, f0, min_fftpoint=1024, max_fftpoint=65536, target_samples=1 << 20, calls=3,
+             method=METHOD_MSOSM, chirp=CHIRP_REF_F32W, form=FORM_FUSED_F16, stream=None):
+    """Measured best FFT size, form and launch shapes for these DM trials:
+    dict(fftpoint, form, tuning (Engine.set_tuning keywords), ns_per_dm_sample,
+    candidates)."""
+    n = max(np.size(bw), np.size(dm), np.size(f0))
+    bwa, dma, f0a = _farr(bw, n), _farr(dm, n), _farr(f0, n)
+    fp = ctypes.POINTER(ctypes.c_float)
+    d = PlanDesc(n, bwa.ctypes.data_as(fp), dma.ctypes.data_as(fp), f0a.ctypes.data_as(fp), 0, 1, 0, method,
+                 chirp, form)
+    r = AutotuneResult()
+    check(lib().msosm_autotune(ctypes.byref(d), min_fftpoint, max_fftpoint, target_samples, calls,
+                               _stream_handle(stream), ctypes.byref(r)))
+    t = r.tuning
+    return dict(fftpoint=int(r.fftpoint), form=int(r.form),
+                tuning=dict(inv_blocks_per_wg=int(t.inv_blocks_per_wg),
+                            hankel_tiles_per_group=int(t.hankel_tiles_per_group),
+                            scratch_chunk_kib=int(t.scratch_chunk_kib)),
+                ns_per_dm_sample=float(r.ns_per_dm_sample), candidates=int(r.candidates))
 
 
 def fold_phase(a, n, period_samples, nbins, prof, hits=None, b=None, ndm=1, stride=None, n0=0, power=True,

@@ -15,6 +15,7 @@
 // msosm_gpu_batch_stream.cpp:105-149); it has no graph and no autotuner.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <map>
 #include <mutex>
 #include <string>
@@ -276,7 +277,8 @@ int msosm_batch_destroy(msosm_batch *b)
 }
 
 // ---------------------------------------------------------------------------
-// Autotuner: measured choice of the FFT size.
+// Autotuner: measured choice of the FFT size, the form it runs with and the
+// launch shapes of that form.
 
 static int ilog2_fp(unsigned long v)
 {
@@ -286,18 +288,51 @@ static int ilog2_fp(unsigned long v)
     return l;
 }
 
-int msosm_autotune_fftpoint(const msosm_plan_desc *desc, unsigned long min_fftpoint, unsigned long max_fftpoint,
-                            unsigned long target_samples, int calls, void *stream, unsigned long *best_fftpoint,
-                            double *best_ns_per_dm_sample, int *best_form)
+// The form a descriptor's form runs with at FFT size 2^log2n: the fused
+// forms go up to 2^21, the matrix-core and Hankel forms up to 2^14 (above,
+// the fused FP16-storage form, the same precision class); beyond 2^21 the
+// reference-structured form.
+static int form_at(int form, int log2n)
 {
-    struct Choice {
-        unsigned long fftpoint;
-        double ns;
-        int form;
-    };
-    static std::map<std::string, Choice> cache;
+    if (form == MSOSM_FORM_PLAIN_F32)
+        return form;
+    const bool single = msosm::fused_supported(log2n);
+    const bool big = msosm::big_supported(log2n);
+    if (form == MSOSM_FORM_FUSED_F32 || form == MSOSM_FORM_FUSED_F16)
+        return (single || big) ? form : MSOSM_FORM_PLAIN_F32;
+    if (single && (form != MSOSM_FORM_FUSED_F16_MFMA || log2n == 14))
+        return form;
+    return big || single ? MSOSM_FORM_FUSED_F16 : MSOSM_FORM_PLAIN_F32;
+}
+
+// Launch-shape candidates of a form at a size (the rule first).
+static std::vector<msosm_tuning> tuning_candidates(int form, int log2n, int count, int ndm)
+{
+    std::vector<msosm_tuning> out(1, msosm_tuning{0, 0, 0});
+    const bool hankel =
+        form == MSOSM_FORM_HANKEL_F16 || form == MSOSM_FORM_HANKEL_BF16 || form == MSOSM_FORM_HANKEL_BF16X3;
+    if (form != MSOSM_FORM_PLAIN_F32 && log2n >= 12 && log2n <= 14) {
+        for (int nb : {4, 8, 16, 32})
+            if (nb <= count)
+                out.push_back(msosm_tuning{nb, 0, 0});
+    }
+    if (hankel) {
+        const int ntiles = (ndm + 15) / 16;
+        for (int t : {1, 2, 4, 8})
+            if (t < ntiles)
+                out.push_back(msosm_tuning{0, t, 0});
+    }
+    if ((form == MSOSM_FORM_FUSED_F32 || form == MSOSM_FORM_FUSED_F16) && msosm::big_supported(log2n))
+        out.push_back(msosm_tuning{0, 0, 64 * 1024});
+    return out;
+}
+
+int msosm_autotune(const msosm_plan_desc *desc, unsigned long min_fftpoint, unsigned long max_fftpoint,
+                   unsigned long target_samples, int calls, void *stream, msosm_autotune_result *result)
+{
+    static std::map<std::string, msosm_autotune_result> cache;
     static std::mutex cache_mu; // held for lookups and inserts, not while timing
-    if (!desc || !best_fftpoint || calls < 1 || target_samples < 1)
+    if (!desc || !result || calls < 1 || target_samples < 1 || desc->num_dms < 1)
         return msosm::set_error(MSOSM_ERR_ARG, "autotune: bad argument");
     std::string key;
     {
@@ -314,18 +349,12 @@ int msosm_autotune_fftpoint(const msosm_plan_desc *desc, unsigned long min_fftpo
         std::lock_guard<std::mutex> lk(cache_mu);
         auto it = cache.find(key);
         if (it != cache.end()) {
-            *best_fftpoint = it->second.fftpoint;
-            if (best_ns_per_dm_sample)
-                *best_ns_per_dm_sample = it->second.ns;
-            if (best_form)
-                *best_form = it->second.form;
+            *result = it->second;
             return MSOSM_OK;
         }
     }
-    hipStream_t s = (hipStream_t)stream;
-    unsigned long best = 0;
-    double best_ns = 1e300;
-    int best_f = desc->form;
+    msosm_autotune_result best{};
+    best.ns_per_dm_sample = 1e300;
     uint32_t *input = nullptr;
     size_t input_words = 0;
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -347,16 +376,14 @@ int msosm_autotune_fftpoint(const msosm_plan_desc *desc, unsigned long min_fftpo
             msosm_plan_destroy(plan);
         }
     }
+    int timed = 0;
     for (unsigned long fp = 4; fp <= max_fftpoint && fp <= (1ul << 28); fp <<= 1) {
         if (fp < min_fftpoint || fp < fp_min)
             continue;
         msosm_plan_desc d = *desc;
         d.fftpoint = fp;
         d.input_size = 0;
-        // the single-workgroup forms stop at 2^14 points; above that the size
-        // is measured with the reference-structured form
-        if (d.form != MSOSM_FORM_PLAIN_F32 && !msosm::fused_supported(ilog2_fp(fp)))
-            d.form = MSOSM_FORM_PLAIN_F32;
+        d.form = form_at(desc->form, ilog2_fp(fp));
         d.count = (int)std::max(1ul, target_samples / (fp / 2));
         msosm_plan *plan = nullptr;
         if (msosm_plan_create(&d, &plan) != MSOSM_OK)
@@ -386,40 +413,61 @@ int msosm_autotune_fftpoint(const msosm_plan_desc *desc, unsigned long min_fftpo
             (void)msosm::launch_fill_random_u16(input, words, 0x5EED, msosm::engine_stream(eng));
         }
         hipStream_t es = msosm::engine_stream(eng);
-        int rc = msosm_engine_filter_u16(eng, input, 1); // warm-up
-        (void)hipEventRecord(e0, es);
-        for (int c = 0; c < calls && rc == MSOSM_OK; c++)
-            rc = msosm_engine_filter_u16(eng, input, 1);
-        (void)hipEventRecord(e1, es);
-        float ms = 0;
-        if (rc == MSOSM_OK && hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(&ms, e0, e1) == hipSuccess) {
-            double ns = ms * 1e6 / ((double)calls * words * info.num_dms);
-            if (ns < best_ns) {
-                best_ns = ns;
-                best = fp;
-                best_f = d.form;
+        for (const msosm_tuning &tn : tuning_candidates(d.form, ilog2_fp(fp), info.count, info.num_dms)) {
+            int rc = msosm_engine_set_tuning(eng, &tn);
+            if (rc == MSOSM_OK)
+                rc = msosm_engine_filter_u16(eng, input, 1); // warm-up
+            (void)hipEventRecord(e0, es);
+            for (int c = 0; c < calls && rc == MSOSM_OK; c++)
+                rc = msosm_engine_filter_u16(eng, input, 1);
+            (void)hipEventRecord(e1, es);
+            float ms = 0;
+            if (rc == MSOSM_OK && hipEventSynchronize(e1) == hipSuccess &&
+                hipEventElapsedTime(&ms, e0, e1) == hipSuccess) {
+                timed++;
+                const double ns = ms * 1e6 / ((double)calls * words * info.num_dms);
+                if (ns < best.ns_per_dm_sample) {
+                    best.ns_per_dm_sample = ns;
+                    best.fftpoint = fp;
+                    best.form = d.form;
+                    best.tuning = tn;
+                }
             }
         }
         msosm_engine_destroy(eng);
         msosm_plan_destroy(plan);
     }
-    (void)s;
     if (input)
         (void)hipFree(input);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    if (!best)
+    if (!best.fftpoint)
         return msosm::set_error(MSOSM_ERR_UNSUPPORTED, "autotune: no FFT size in [%lu, %lu] fits these DM trials",
                                 min_fftpoint, max_fftpoint);
+    best.candidates = timed;
     {
         std::lock_guard<std::mutex> lk(cache_mu);
-        cache[key] = {best, best_ns, best_f};
+        cache[key] = best;
     }
-    *best_fftpoint = best;
+    *result = best;
+    return MSOSM_OK;
+}
+
+int msosm_autotune_fftpoint(const msosm_plan_desc *desc, unsigned long min_fftpoint, unsigned long max_fftpoint,
+                            unsigned long target_samples, int calls, void *stream, unsigned long *best_fftpoint,
+                            double *best_ns_per_dm_sample, int *best_form)
+{
+    if (!best_fftpoint)
+        return msosm::set_error(MSOSM_ERR_ARG, "autotune: bad argument");
+    msosm_autotune_result r{};
+    int rc = msosm_autotune(desc, min_fftpoint, max_fftpoint, target_samples, calls, stream, &r);
+    if (rc != MSOSM_OK)
+        return rc;
+    *best_fftpoint = r.fftpoint;
     if (best_ns_per_dm_sample)
-        *best_ns_per_dm_sample = best_ns;
+        *best_ns_per_dm_sample = r.ns_per_dm_sample;
     if (best_form)
-        *best_form = best_f;
+        *best_form = r.form;
     return MSOSM_OK;
 }
 

@@ -105,8 +105,42 @@ def test_autotune_picks_a_valid_size(device):
 
 
 def test_autotune_above_single_workgroup_sizes(device):
-    # sizes above 2^14 are timed with the reference-structured form
+    # sizes above 2^14 run the four-step fused form; above 2^21 the
+    # reference-structured form
     bw, f0 = 128e6, 1336e6
     best, ns, form = ms.autotune_fftpoint(bw, [100.0], f0, min_fftpoint=32768, max_fftpoint=65536,
                                           target_samples=1 << 20, calls=1)
-    assert best in (32768, 65536) and form == ms.FORM_PLAIN_F32 and ns > 0
+    assert best in (32768, 65536) and form == ms.FORM_FUSED_F16 and ns > 0
+    r = ms.autotune(bw, [100.0], f0, min_fftpoint=1 << 22, max_fftpoint=1 << 22, target_samples=1 << 22, calls=1,
+                    form=ms.FORM_FUSED_F32)
+    assert r["fftpoint"] == 1 << 22 and r["form"] == ms.FORM_PLAIN_F32
+
+
+def test_autotune_launch_shapes(device):
+    """The tuner times the launch shapes of each size: blocks per workgroup
+    for the 2^12..2^14 inverses, tiles per group for the Hankel GEMM; the
+    chosen shapes run and give the rule's output bit for bit."""
+    import torch
+    bw, f0 = 64e6, 1368e6
+    dms = list(np.linspace(0, 60, 40))
+    for form in (ms.FORM_FUSED_F16, ms.FORM_HANKEL_F16):
+        r = ms.autotune(bw, dms, f0, min_fftpoint=8192, max_fftpoint=16384, target_samples=1 << 18, calls=2,
+                        form=form)
+        print(f"form {form}: {r}", flush=True)
+        assert r["form"] == form and r["fftpoint"] in (8192, 16384)
+        assert r["candidates"] >= 6
+        p = ms.Plan(bw, dms, f0, fftpoint=r["fftpoint"], count=16, form=form)
+        n = p.count * p.M
+        rng = np.random.Generator(np.random.Philox(5))
+        u = np.clip(np.round(32768 + 800 * rng.standard_normal(4 * n)), 1, 65535).astype(np.uint16)
+        dev = torch.from_numpy(u.view(np.int32)).to(device)
+        outs = []
+        for tuned in (False, True):
+            e = ms.Engine(p)
+            if tuned:
+                e.set_tuning(**r["tuning"])
+            for c in range(2):
+                e.filter_u16(dev[c * n:(c + 1) * n])
+            outs.append(e.output_c64())
+            e.close()
+        np.testing.assert_array_equal(outs[0], outs[1])

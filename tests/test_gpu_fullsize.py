@@ -164,17 +164,20 @@ def test_uneven_batch_24_streams(device):
         ndm = int(rng.integers(1, 65))
         dms = np.sort(rng.uniform(0, 300, ndm)).astype(np.float32)
         period = float(np.exp(rng.uniform(np.log(1e-3), np.log(1.0))))
-        fp, ns, form = ms.autotune_fftpoint(bw, dms, f0, min_fftpoint=1024, max_fftpoint=65536,
-                                            target_samples=min(L, 1 << 19), calls=1, form=ms.FORM_FUSED_F16)
+        tuned = ms.autotune(bw, dms, f0, min_fftpoint=1024, max_fftpoint=65536, target_samples=min(L, 1 << 19),
+                            calls=1, form=ms.FORM_FUSED_F16)
+        fp, form = tuned["fftpoint"], tuned["form"]
         M = fp // 2
         count = max(1, min(64, L // M))
-        streams.append(dict(bw=bw, f0=f0, L=L, dms=dms, period=period * bw, fp=fp, form=form, count=count))
+        streams.append(dict(bw=bw, f0=f0, L=L, dms=dms, period=period * bw, fp=fp, form=form, count=count,
+                            tuning=tuned["tuning"]))
         print(f"stream {s}: {bw / 1e6:.0f} MHz, 2^{int(np.log2(L))} samples, {ndm} trials, FFT {fp} "
               f"(form {form}), P {period * 1e3:.1f} ms", flush=True)
     engines, inputs = [], []
     for k, st in enumerate(streams):
         p = ms.Plan(st["bw"], st["dms"], st["f0"], fftpoint=st["fp"], count=st["count"], form=st["form"])
         engines.append(ms.Engine(p))
+        engines[-1].set_tuning(**st["tuning"])
         inputs.append(device_input(st["L"], 1000 + k, device))
         st["plan"] = p
     torch.cuda.synchronize()
