@@ -5,10 +5,23 @@
 // Output handshake: the reference copies count*M*numDMs + 1 elements, the
 // last being a sentinel a consumer thread spins on, and guards its host
 // buffer with plain bool flags (msosm_gpu_dm_concurrent.cpp:153-195).  The
-// facade keeps the same members and the sentinel, makes the flags atomic and
-// returns from get_output only once the copy is complete.
+// facade keeps the same members and the sentinel and makes the flags atomic.
+// get_output returns once the copy is complete.  get_output_thread (one
+// writer thread per buffer, multiple_dm_concurrent_msosm.cpp:117-146) hands
+// the output over asynchronously through the library's output ring
+// (msosm_outring_*): the call pushes this call's output into a ring slot (a
+// copy on the engine's stream, then to page-locked memory on the ring's
+// stream) and returns, so the engine's next call can start at once; a
+// delivery thread takes the slots in push order, copies each into the
+// buffer given with it and only then clears that buffer's sentinel (a
+// release store), so a writer never sees the sentinel before the data.
 #pragma once
+#include <condition_variable>
+#include <cstring>
+#include <deque>
 #include <memory>
+#include <mutex>
+#include <thread>
 #include <vector>
 
 #include "globals.h"
@@ -31,7 +44,11 @@ public:
         delaycount = K_.data();
         wait_for_cpu = false;
     }
-    ~MSOSM_GPU_DM_concurrent() { core_.destroy(); }
+    ~MSOSM_GPU_DM_concurrent()
+    {
+        stop_delivery();
+        core_.destroy();
+    }
     MSOSM_GPU_DM_concurrent(const MSOSM_GPU_DM_concurrent &) = delete;
     MSOSM_GPU_DM_concurrent &operator=(const MSOSM_GPU_DM_concurrent &) = delete;
 
@@ -40,6 +57,7 @@ public:
     void initialize_uint16(unsigned long fftpoint, int count = 1, unsigned long input_size = 0,
                            bool compute_only = false)
     {
+        stop_delivery();
         core_.destroy();
         core_.create(numDMs, bw_.data(), dm_.data(), f0_.data(), fftpoint, count, input_size, MSOSM_METHOD_MSOSM);
         M_common = core_.info.M;
@@ -52,6 +70,7 @@ public:
     // msosm_gpu_dm_concurrent.cpp:110-117
     void initialize_threads_flag(int thread_num)
     {
+        nthreads_ = thread_num;
         wait_for_threads_.reset(new std::atomic<bool>[thread_num]);
         for (int i = 0; i < thread_num; i++)
             wait_for_threads_[i] = false;
@@ -80,14 +99,40 @@ public:
         msosm_facade::check(msosm_engine_output_u16(core_.eng, output, 0), "get_output");
         output[total()] = uint16_pair{0, 0};
     }
-    // msosm_gpu_dm_concurrent.cpp:182-195
+    // msosm_gpu_dm_concurrent.cpp:182-195: asynchronous hand-over through
+    // the output ring (see the header comment); the sentinel output[total]
+    // reads 0 once the data is in `output`
     void get_output_thread(uint16_pair *output, int thread_id)
     {
         acquire(wait_for_threads[thread_id]);
-        msosm_facade::check(msosm_engine_output_u16(core_.eng, output, 0), "get_output_thread");
-        output[total()] = uint16_pair{0, 0};
+        start_delivery();
+        {
+            std::lock_guard<std::mutex> lk(dmu_);
+            dest_.push_back(output); // before the push: the delivery thread pops in push order
+            pending_++;
+        }
+        long long seq = 0;
+        const int rc = msosm_outring_push(ring_, core_.eng, 1, -1, &seq);
+        if (rc != MSOSM_OK) {
+            // nothing was queued: take the destination back (this thread is
+            // the only producer, so it is the last entry)
+            std::lock_guard<std::mutex> lk(dmu_);
+            dest_.pop_back();
+            pending_--;
+            wait_for_threads[thread_id] = false;
+        }
+        msosm_facade::check(rc, "get_output_thread");
     }
-    void synchronize() { msosm_facade::check(msosm_engine_synchronize(core_.eng), "synchronize"); }
+    // waits for the engine and for every output handed to writer threads to
+    // be in its buffer (the reference's cudaDeviceSynchronize covers both)
+    void synchronize()
+    {
+        msosm_facade::check(msosm_engine_synchronize(core_.eng), "synchronize");
+        std::unique_lock<std::mutex> lk(dmu_);
+        dcv_.wait(lk, [&] { return pending_ == 0 || delivery_error_; });
+        if (delivery_error_)
+            msosm_facade::check(MSOSM_ERR_HIP, "output delivery");
+    }
     // The reference calls cudaDeviceReset (msosm_gpu_dm_concurrent.cpp:202-205),
     // which tears down every allocation of the process; here only this
     // engine's stream history is cleared.
@@ -114,12 +159,83 @@ private:
                 expect = false;
         }
     }
+    // ring of (writer threads + 1) slots and the thread that empties it
+    void start_delivery()
+    {
+        if (ring_)
+            return;
+        int threads = 1;
+        if (wait_for_threads_)
+            threads = nthreads_;
+        msosm_facade::check(msosm_outring_create(sizeof(uint16_pair) * total(), threads + 1, &ring_),
+                            "output ring");
+        stop_ = false;
+        delivery_ = std::thread([this] { deliver(); });
+    }
+    void deliver()
+    {
+        const size_t n = total();
+        for (;;) {
+            const void *data = nullptr;
+            size_t bytes = 0;
+            long long seq = 0;
+            const int rc = msosm_outring_pop(ring_, 50, &data, &bytes, &seq);
+            if (rc != MSOSM_OK) {
+                std::lock_guard<std::mutex> lk(dmu_);
+                if (rc != MSOSM_ERR_STATE) { // not a time-out
+                    delivery_error_ = true;
+                    dcv_.notify_all();
+                    return;
+                }
+                if (stop_ && pending_ == 0)
+                    return;
+                continue;
+            }
+            uint16_pair *dst = nullptr;
+            {
+                std::lock_guard<std::mutex> lk(dmu_);
+                dst = dest_.front();
+                dest_.pop_front();
+            }
+            std::memcpy(dst, data, bytes);
+            __atomic_store_n(&dst[n].second, (uint16_t)0, __ATOMIC_RELAXED);
+            __atomic_store_n(&dst[n].first, (uint16_t)0, __ATOMIC_RELEASE); // after the data
+            msosm_outring_release(ring_);
+            std::lock_guard<std::mutex> lk(dmu_);
+            pending_--;
+            dcv_.notify_all();
+        }
+    }
+    void stop_delivery()
+    {
+        if (!ring_)
+            return;
+        {
+            std::unique_lock<std::mutex> lk(dmu_);
+            dcv_.wait(lk, [&] { return pending_ == 0 || delivery_error_; });
+            stop_ = true;
+        }
+        delivery_.join();
+        msosm_outring_destroy(ring_);
+        ring_ = nullptr;
+        dest_.clear();
+        pending_ = 0;
+        delivery_error_ = false;
+    }
     bool compute_only = false;
     std::vector<float> bw_, dm_, f0_;
     std::vector<unsigned long> Nd_, M_min_;
     std::vector<int> K_;
     std::unique_ptr<std::atomic<bool>[]> wait_for_threads_;
+    int nthreads_ = 0;
     msosm_facade::Core core_;
+    msosm_outring *ring_ = nullptr;
+    std::thread delivery_;
+    std::mutex dmu_;
+    std::condition_variable dcv_;
+    std::deque<uint16_pair *> dest_;
+    long long pending_ = 0;
+    bool stop_ = false, delivery_error_ = false;
 };
 
 // dedispersion/include/msosm_gpu_batch_stream.h:52-93: many DM trials of one

@@ -94,7 +94,7 @@ def build_facade_check():
     """C++ program written against the drop-in facade headers (tests/cpp)."""
     src = os.path.join(ROOT, "tests", "cpp", "facade_check.cpp")
     out = os.path.join(ROOT, "tests", "cpp", "facade_check")
-    _run(["g++", "-std=c++17", "-O2", "-Wall", "-I", INCLUDE, src, "-o", out, "-L", LIBDIR, "-lmsosm",
+    _run(["g++", "-std=c++17", "-O2", "-Wall", "-pthread", "-I", INCLUDE, src, "-o", out, "-L", LIBDIR, "-lmsosm",
           "-Wl,-rpath,$ORIGIN/../../msosm_gpu_amd/lib"])
     return out
 

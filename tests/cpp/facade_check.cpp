@@ -3,12 +3,17 @@
 // multiple_dm_concurrent_msosm.cpp, msosm_psrdada.cpp) and writes every
 // output to files that tests/test_gpu_facade.py checks against the oracle.
 //
-// usage: facade_check <input.u16> <num_samples> <out_prefix> [default_plan]
+// usage: facade_check <input.u16> <num_samples> <out_prefix> [default_plan | writer_threads]
 //   default_plan: only the single-DM engine at the reference recording
 //   program's plan (fftpoint 2097152, count 1, msosm_psrdada.cpp:94-95) at
 //   128 MHz, DM 100, f0 1336 MHz
+//   writer_threads: only the concurrent engine with two writer threads fed
+//   by get_output_thread, alternating by call, each writer spinning on its
+//   buffer's sentinel and appending what it gets to its own file, as
+//   multiple_dm_concurrent_msosm.cpp:117-146 does
 #include <cstdio>
 #include <fstream>
+#include <thread>
 #include <vector>
 
 #include "msosm/fold_gpu.h"
@@ -55,6 +60,44 @@ int main(int argc, char **argv)
             big.get_output(out.data());
             dump(pre + ".big.c64", out.data(), out.size());
         }
+        return 0;
+    }
+    if (argc >= 5 && std::string(argv[4]) == "writer_threads") {
+        float bws[3] = {16e6f, 16e6f, 16e6f}, dms[3] = {5.0f, 10.0f, 20.0f}, f0s[3] = {1392e6f, 1392e6f, 1392e6f};
+        MSOSM_GPU_DM_concurrent conc(bws, dms, f0s, 3);
+        conc.initialize_uint16(1024, 8);
+        conc.initialize_threads_flag(2);
+        const size_t tot = (size_t)conc.count * conc.M_common * 3;
+        const unsigned long per = (unsigned long)conc.count * conc.M_common;
+        const int calls = (int)(L / per);
+        std::vector<uint16_pair> buf[2] = {std::vector<uint16_pair>(tot + 1), std::vector<uint16_pair>(tot + 1)};
+        std::vector<std::thread> writers;
+        for (int id = 0; id < 2; id++) {
+            std::remove((pre + ".thread" + std::to_string(id) + ".u16").c_str());
+            buf[id][tot].first = 1;
+            const int mine = (calls - id + 1) / 2;
+            writers.emplace_back([&, id, mine] {
+                uint16_pair *src = buf[id].data();
+                for (int k = 0; k < mine; k++) {
+                    while (__atomic_load_n(&src[tot].first, __ATOMIC_ACQUIRE) == 1)
+                        ;
+                    PUSH_RANGE("Output Copy", id)
+                    __atomic_store_n(&src[tot].first, (uint16_t)1, __ATOMIC_RELAXED);
+                    dump(pre + ".thread" + std::to_string(id) + ".u16", src, tot);
+                    conc.wait_for_threads[id] = false;
+                    POP_RANGE
+                }
+            });
+        }
+        for (int c = 0; c < calls; c++) {
+            conc.filter_block_uint16(in.data() + (size_t)c * per);
+            conc.get_output_thread(buf[c % 2].data(), c % 2);
+        }
+        for (auto &w : writers)
+            w.join();
+        conc.synchronize();
+        std::ofstream meta(pre + ".meta");
+        meta << "writer_threads calls " << calls << "\n";
         return 0;
     }
     for (const char *s : {".batch.c64", ".conc.c64", ".conc.u16", ".osm.c64", ".loop.c64", ".fold.f32", ".inten.f32",

@@ -114,3 +114,31 @@ def test_facade_reference_default_plan(device, tmp_path):
     err = rel_l2(y, ref)
     print(f"facade at FFT 2^21, count 1: rel L2 vs FP64 {err:.2e}")
     assert err < 1e-5
+
+
+def test_facade_writer_threads(device, tmp_path):
+    """get_output_thread with two writer threads (the reference driver's
+    multiple_dm_concurrent_msosm.cpp:117-146 pattern): the outputs go through
+    the output ring asynchronously; every call's output reaches its writer
+    exactly once, in call order, equal to the synchronous get_output."""
+    if not os.path.exists(BIN):
+        pytest.fail("tests/cpp/facade_check is not built (run __graft_entry__.build())")
+    rng = np.random.Generator(np.random.Philox(79))
+    L = 1 << 17
+    u = np.clip(np.round(32768 + 900 * rng.standard_normal(2 * L)), 1, 65535).astype(np.uint16)
+    inp = tmp_path / "in.u16"
+    u.tofile(inp)
+    pre_t, pre_s = str(tmp_path / "thr"), str(tmp_path / "sync")
+    for pre, mode in ((pre_t, ["writer_threads"]), (pre_s, [])):
+        r = subprocess.run([BIN, str(inp), str(L), pre] + mode, capture_output=True, text=True, timeout=240)
+        print(r.stdout, r.stderr)
+        assert r.returncode == 0, r.stderr
+    calls = L // (8 * 512)
+    assert f"writer_threads calls {calls}" in open(pre_t + ".meta").read()
+    per = 3 * 8 * 512 * 2                       # uint16 values per call (3 trials)
+    sync = np.fromfile(pre_s + ".conc.u16", np.uint16).reshape(calls, per)
+    for tid in (0, 1):
+        got = np.fromfile(pre_t + f".thread{tid}.u16", np.uint16)
+        want = sync[tid::2].reshape(-1)
+        assert got.size == want.size, (tid, got.size, want.size)
+        np.testing.assert_array_equal(got, want)
