@@ -14,14 +14,23 @@ the matrix-core (per-bin Hankel GEMM) forms for comparison.  Every form is
 checked against the FP64 same-plan oracle on a block range.  Per-kernel
 rooflines: the dominant kernel of the headline form, the forward FFT, the
 plain form's inverse FFT, the GEMM kernel (useful and issued MFMA flops) and
-the phase fold over the output.
+the phase fold over the output.  HBM traffic per launch and the matrix-core
+counters come from hardware counters collected live: three rocprofv3 passes
+over a child process that replays a short slice of the same workload.
 
 Metric: DM-trial * Msamples / s = num_dms * input_samples / time / 1e6.
 """
 import argparse
+import csv
+import glob
 import json
 import os
+import re
+import shutil
+import signal
+import subprocess
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -50,8 +59,11 @@ def parse():
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP32 comparison")
     ap.add_argument("--fold-bins", type=int, default=512)
     ap.add_argument("--pmc-workload", action="store_true",
-                    help="short run for hardware-counter passes: a few calls of the headline form and a 1 GiB "
-                         "device copy (the counter calibration), no timing")
+                    help="short run for hardware-counter passes: a few calls of each --pmc-forms form, phase "
+                         "folds of the last output and a 1 GiB device copy (the counter calibration), no timing")
+    ap.add_argument("--pmc-forms", default="mixed_f16", help="comma list of forms the counter run drives")
+    ap.add_argument("--no-counters", action="store_true",
+                    help="skip the rocprofv3 counter passes (traffic and matrix-core counters are then null)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-check", action="store_true",
                     help="skip the FP64 check of every form (and the comparison with the plain FP32 output)")
@@ -176,6 +188,12 @@ def main():
         t1 = time.time()
         osm = osm_informational(ms, bw, f0, dms, host_reader(u), mixed_rows, fp64_ref, plan_info["M"])
         log(f"against FP64 single-segment OSM ({time.time() - t1:.1f}s): {json.dumps(osm)}")
+    counters, counters_note = None, "skipped (--no-counters)"
+    if not args.no_counters:
+        t1 = time.time()
+        counters, counters_note = counter_passes(args, ",".join(f for f in wanted if f != "plain_f32"))
+        log(f"counter passes ({time.time() - t1:.0f}s): {counters_note or 'ok'}")
+    head_traffic = hbm_traffic(counters, "k_inv_fused_nb<") if counters else None
     mixed = results["mixed_f16"]
     st = mixed["stages"]["fft_inv"]
     units = len(dms) * plan_info["count"]
@@ -184,12 +202,15 @@ def main():
     kname = f"k_inv_fused_nb<{(2 * plan_info['M']).bit_length() - 1},fp16>"
     roofline = dict(bound="hbm", kernel=kname, achieved=round(achieved, 1), peak=HBM_PEAK_GBS,
                     unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 4),
-                    traffic=measured_traffic(args, len(dms)),
+                    traffic=head_traffic["total"] if head_traffic else None,
+                    traffic_read_write=[head_traffic["read"], head_traffic["written"]] if head_traffic else None,
                     alg_bytes_per_launch=alg_bytes, ms_per_launch=st["ms_per_launch"])
-    kernels = kernel_rooflines(results, plan_info, len(dms), fold, hankel_flops)
+    kernels = kernel_rooflines(results, plan_info, len(dms), fold, hankel_flops, counters)
     copy_gbs = copy_bandwidth(torch, dev)
     log(f"device copy (1 GiB read + 1 GiB write): {copy_gbs:.0f} GB/s")
     kernels[kname] = {k: roofline[k] for k in ("achieved", "unit", "peak", "frac", "ms_per_launch")}
+    if head_traffic:
+        kernels[kname]["traffic_bytes"] = head_traffic["total"]
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(bw, f0, dms, args.fftpoint // 2, u, args.cpu_log2_samples)
@@ -224,6 +245,10 @@ def main():
         "ratio_mixed_vs_plain_f32": (round(mixed["value"] / results["plain_f32"]["value"], 3)
                                      if "plain_f32" in results else None),
         "kernels": kernels,
+        "counters": {"source": "rocprofv3 --pmc passes over a 6-call slice of each form in this run",
+                     "note": counters_note,
+                     "calibration": ({k: head_traffic[k] for k in ("fetch_factor", "write_factor")}
+                                     if head_traffic else None)},
         "hbm_copy_gbs": round(copy_gbs, 1),
         "host_input_pcie": pcie,
         "rel_l2_vs_fp64": fp64 or None,
@@ -392,45 +417,131 @@ def other_configs(ms, torch, dev):
     return out
 
 
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_multidm_2e14_256dm.md")
+# counter passes: one block of counters per rocprofv3 run (FETCH_SIZE and
+# WRITE_SIZE each fill most of the TCC block; six SQ counters)
+COUNTER_SETS = (("FETCH_SIZE",), ("WRITE_SIZE",),
+                ("SQ_INSTS_MFMA", "SQ_VALU_MFMA_BUSY_CYCLES", "SQ_BUSY_CYCLES", "SQ_INSTS_VALU_MFMA_MOPS_F16",
+                 "SQ_INSTS_VALU_MFMA_MOPS_BF16", "SQ_WAVES"))
 
 
-def measured_traffic(args, ndm):
-    """HBM bytes per launch of the fused inverse from the committed counter
-    summary (FETCH_SIZE and WRITE_SIZE passes scaled by a 1 GiB copy); only
-    for the workload it was measured on."""
-    if (ndm, args.count, args.fftpoint) != (256, 64, 16384) or not os.path.exists(TRAFFIC_FILE):
+def _kernel_key(name):
+    """Short kernel name: template arguments kept, parameter list dropped."""
+    n = name.replace("(anonymous namespace)::", "").replace("void ", "")
+    return re.sub(r"\(.*", "", n).replace("msosm::", "").strip()
+
+
+def counter_passes(args, forms, timeout_s=150):
+    """Hardware counters per dispatch for the kernels of `forms` and the phase
+    fold: one rocprofv3 --pmc pass per counter set over a child process that
+    runs --pmc-workload with this run's workload sizes.  Returns (counters,
+    None) with counters = {kernel: {counter: mean per dispatch}} plus the
+    calibration copy's FETCH_SIZE and WRITE_SIZE, or (None, reason)."""
+    exe = shutil.which("rocprofv3")
+    if exe is None:
+        return None, "rocprofv3 not on PATH"
+    out = {}
+    with tempfile.TemporaryDirectory(dir="/tmp", prefix="msosm_pmc_") as td:
+        for i, cs in enumerate(COUNTER_SETS):
+            d = os.path.join(td, f"p{i}")
+            cmd = [exe, "--pmc", *cs, "--output-format", "csv", "-d", d, "-o", "pmc", "--", sys.executable, "-u",
+                   os.path.abspath(__file__), "--pmc-workload", "--pmc-forms", forms, "--num-dms", str(args.num_dms),
+                   "--count", str(args.count), "--fftpoint", str(args.fftpoint), "--fold-bins", str(args.fold_bins)]
+            log(f"counter pass {i + 1}/{len(COUNTER_SETS)}: {' '.join(cs)}")
+            with open(os.path.join(td, f"p{i}.log"), "w") as lf:
+                p = subprocess.Popen(cmd, cwd="/tmp", env=dict(os.environ, TMPDIR="/tmp"), stdout=lf,
+                                     stderr=subprocess.STDOUT, start_new_session=True)
+                try:
+                    rc = p.wait(timeout=timeout_s)
+                except subprocess.TimeoutExpired:
+                    os.killpg(p.pid, signal.SIGKILL)
+                    p.wait()
+                    return None, f"counter pass {' '.join(cs)} exceeded {timeout_s} s"
+            text = open(os.path.join(td, f"p{i}.log")).read()
+            files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
+            # the profiler may end with its own fault after the program has
+            # printed its last line and the files are whole: accept that
+            if not files or '"pmc_workload"' not in text:
+                return None, f"counter pass {' '.join(cs)} failed (rc {rc}): {text[-300:]}"
+            rows = list(csv.DictReader(open(files[0])))
+            copies = [r for r in rows if "copyBuffer" in r["Kernel_Name"]]
+            big = max((int(r["Grid_Size"]) for r in copies), default=0)
+            for r in rows:
+                k = "calibration copy" if ("copyBuffer" in r["Kernel_Name"] and int(r["Grid_Size"]) == big) \
+                    else _kernel_key(r["Kernel_Name"])
+                out.setdefault(k, {}).setdefault(r["Counter_Name"], []).append(float(r["Counter_Value"]))
+    return {k: {c: sum(v) / len(v) for c, v in cs.items()} for k, cs in out.items()}, None
+
+
+def hbm_traffic(counters, prefix, copy_bytes=1 << 30):
+    """HBM bytes per dispatch of the kernels whose short name starts with
+    `prefix`: FETCH_SIZE and WRITE_SIZE (KiB) divided by what the same
+    counters read for the 1 GiB calibration copy (read and write 1 GiB)."""
+    cal = counters.get("calibration copy", {})
+    if not cal.get("FETCH_SIZE") or not cal.get("WRITE_SIZE"):
         return None
-    for line in open(TRAFFIC_FILE):
-        if line.startswith("| `k_inv_fused_nb"):
-            total = line.split("|")[6].strip().strip("*").split()[0]
-            return int(float(total) * 1e9)
-    return None
+    fetch = [v["FETCH_SIZE"] for k, v in counters.items() if k.startswith(prefix) and "FETCH_SIZE" in v]
+    write = [v["WRITE_SIZE"] for k, v in counters.items() if k.startswith(prefix) and "WRITE_SIZE" in v]
+    if not fetch or not write:
+        return None
+    kib = copy_bytes / 1024
+    rf, wf = cal["FETCH_SIZE"] / kib, cal["WRITE_SIZE"] / kib
+    read, written = fetch[0] * 1024 / rf, write[0] * 1024 / wf
+    return dict(read=int(read), written=int(written), total=int(read + written), fetch_factor=round(rf, 4),
+                write_factor=round(wf, 4))
 
 
-def pmc_workload(ms, torch, dev, bw, f0, dms, args, calls=6):
-    """Dispatches for counter passes (FETCH_SIZE, WRITE_SIZE): `calls` calls
-    of the headline form on random offset-binary input, then two copies of
-    1 GiB between device buffers -- a kernel whose HBM bytes are known
-    (read 1 GiB, write 1 GiB), giving the counters' scale on this device."""
-    plan = ms.Plan(bw, dms, f0, fftpoint=args.fftpoint, count=args.count, form=ms.FORM_FUSED_F16)
-    n = plan.count * plan.M
+def mfma_counters(counters, key, num_se=32, simds_per_se=32):
+    """Matrix-core counters per dispatch of kernel `key`: MFMA instructions,
+    counted flops (MOPS counters x 512), and the MFMA busy fraction =
+    SQ_VALU_MFMA_BUSY_CYCLES / (SIMD count x per-engine busy cycles), the
+    latter SQ_BUSY_CYCLES summed over the 32 shader engines / 32."""
+    c = counters.get(key)
+    if not c or "SQ_INSTS_MFMA" not in c:
+        return None
+    flops = 512 * (c.get("SQ_INSTS_VALU_MFMA_MOPS_F16", 0) + c.get("SQ_INSTS_VALU_MFMA_MOPS_BF16", 0))
+    busy = c["SQ_VALU_MFMA_BUSY_CYCLES"] / (num_se * simds_per_se * c["SQ_BUSY_CYCLES"] / num_se)
+    return dict(mfma_insts=int(c["SQ_INSTS_MFMA"]), mfma_flops_counted=int(flops), mfma_busy_frac=round(busy, 4))
+
+
+def pmc_workload(ms, torch, dev, bw, f0, dms, args, calls=6, folds=4):
+    """Dispatches for counter passes (FETCH_SIZE, WRITE_SIZE, SQ_*): for each
+    form of --pmc-forms, `calls` calls on random offset-binary input and
+    `folds` phase folds of the last output (one launch per form: every trial,
+    count*M samples, --fold-bins bins); then two copies of 1 GiB between
+    device buffers -- a kernel whose HBM bytes are known (read 1 GiB, write
+    1 GiB), giving the counters' scale on this device."""
+    form_ids = {"plain_f32": ms.FORM_PLAIN_F32, "mixed_f16": ms.FORM_FUSED_F16,
+                "mixed_f16_mfma": ms.FORM_FUSED_F16_MFMA, "hankel_f16": ms.FORM_HANKEL_F16,
+                "hankel_bf16x3": ms.FORM_HANKEL_BF16X3}
     g = torch.Generator(device=dev).manual_seed(7)
-    lo = torch.randint(1, 65536, (calls * n,), generator=g, device=dev, dtype=torch.int32)
-    hi = torch.randint(1, 65536, (calls * n,), generator=g, device=dev, dtype=torch.int32)
-    d_in = lo | (hi << 16)
-    eng = ms.Engine(plan, stream=torch.cuda.current_stream())
-    for c in range(calls):
-        eng.filter_u16(d_in[c * n:(c + 1) * n])
-    torch.cuda.synchronize()
-    eng.close()
+    for name in [f for f in args.pmc_forms.split(",") if f]:
+        plan = ms.Plan(bw, dms, f0, fftpoint=args.fftpoint, count=args.count, form=form_ids[name])
+        n = plan.count * plan.M
+        lo = torch.randint(1, 65536, (calls * n,), generator=g, device=dev, dtype=torch.int32)
+        hi = torch.randint(1, 65536, (calls * n,), generator=g, device=dev, dtype=torch.int32)
+        d_in = lo | (hi << 16)
+        del lo, hi
+        stream = torch.cuda.current_stream()
+        eng = ms.Engine(plan, stream=stream)
+        print(f"pmc workload: {name}, {calls} calls", flush=True)
+        for c in range(calls):
+            eng.filter_u16(d_in[c * n:(c + 1) * n])
+        prof = torch.zeros((plan.num_dms, args.fold_bins), dtype=torch.float32, device=dev)
+        hits = torch.zeros(args.fold_bins, dtype=torch.float32, device=dev)
+        for i in range(folds):
+            ms.fold_phase(eng.output_ptr(), n, 0.008 * bw, args.fold_bins, prof, hits, ndm=plan.num_dms, stride=n,
+                          n0=i * n, stream=stream)
+        torch.cuda.synchronize()
+        eng.close()
+        del d_in
     src = torch.ones(1 << 28, dtype=torch.float32, device=dev)
     dst = torch.empty_like(src)
     for _ in range(2):
         dst.copy_(src)
     torch.cuda.synchronize()
-    print(json.dumps({"pmc_workload": {"calls": calls, "units_per_call": len(dms) * plan.count,
-                                       "copy_bytes": src.numel() * 4}}), flush=True)
+    print(json.dumps({"pmc_workload": {"forms": args.pmc_forms, "calls": calls, "folds": folds,
+                                       "units_per_call": len(dms) * plan.count, "copy_bytes": src.numel() * 4}}),
+          flush=True)
 
 
 def copy_bandwidth(torch, dev, reps=5):
@@ -517,21 +628,32 @@ def hankel_flop_count(plan):
     return dict(useful=useful, issued_per_term=issued_per_term, dense=8 * plan.K_max * ndm * N * count)
 
 
-def kernel_rooflines(results, plan_info, ndm, fold, hankel_flops):
+def kernel_rooflines(results, plan_info, ndm, fold, hankel_flops, counters=None):
+    """Per-kernel rooflines from the HIP-event stage times; where the counter
+    passes ran, each entry also carries its counted HBM bytes per launch
+    (traffic_bytes) and, for matrix-core kernels, the MFMA counters."""
     M, count = plan_info["M"], plan_info["count"]
     N = 2 * M
     out = {}
 
-    def put(name, alg, ms_launch, unit="GB/s", peak=HBM_PEAK_GBS, **extra):
+    def put(name, alg, ms_launch, unit="GB/s", peak=HBM_PEAK_GBS, ckey=None, **extra):
         if not ms_launch:
             return
         a = alg / (ms_launch / 1e3) / (1e9 if unit == "GB/s" else 1e12)
         out[name] = dict(achieved=round(a, 1), unit=unit, peak=peak, frac=round(a / peak, 4),
                          ms_per_launch=round(ms_launch, 4), **extra)
+        if counters and ckey:
+            tr = hbm_traffic(counters, ckey)
+            if tr:
+                out[name]["traffic_bytes"] = tr["total"]
+            mf = mfma_counters(counters, ckey)
+            if mf and mf["mfma_insts"]:
+                out[name].update(mf)
+                out[name]["mfma_flops_counted_tflops"] = round(mf["mfma_flops_counted"] / (ms_launch / 1e3) / 1e12, 1)
 
     st = results["mixed_f16"]["stages"]
     # forward FFT: count windows of N points, uint16 in (4 B) and FP16 out (4 B)
-    put("k_fwd_u16<14,fp16>", count * N * 8, st["fft_fwd"]["ms_per_launch"])
+    put("k_fwd_u16<14,fp16>", count * N * 8, st["fft_fwd"]["ms_per_launch"], ckey="k_fwd_u16<14, true, false>")
     if "plain_f32" in results:
         sp = results["plain_f32"]["stages"]
         put("k_fft_c2c<14,inverse> (plain)", ndm * count * N * 16, sp["fft_inv"]["ms_per_launch"])
@@ -540,21 +662,22 @@ def kernel_rooflines(results, plan_info, ndm, fold, hankel_flops):
     if "mixed_f16_mfma" in results:
         # same algorithmic bytes as the vector-ALU fused inverse
         put("k_inv_mfma<14> (matrix-core DFTs)", ndm * count * M * 16,
-            results["mixed_f16_mfma"]["stages"]["fft_inv"]["ms_per_launch"])
-    for name, mode, terms in (("hankel_f16", "fp16", 1), ("hankel_bf16x3", "bf16x3", 3)):
+            results["mixed_f16_mfma"]["stages"]["fft_inv"]["ms_per_launch"], ckey="k_inv_mfma")
+    for name, mode, terms, tmpl in (("hankel_f16", "fp16", 1, 0), ("hankel_bf16x3", "bf16x3", 3, 2)):
         if name not in results or hankel_flops is None:
             continue
         sh = results[name]["stages"]
         t = sh["multiply"]["ms_per_launch"]
         issued = hankel_flops["issued_per_term"] * terms
         put(f"k_hankel<{mode}> (useful flops)", hankel_flops["useful"], t, unit="TFLOP/s",
-            peak=MFMA_F16_PEAK_TFLOPS, issued_tflops=round(issued / (t / 1e3) / 1e12, 1),
+            peak=MFMA_F16_PEAK_TFLOPS, ckey=f"k_hankel<{tmpl}>", issued_tflops=round(issued / (t / 1e3) / 1e12, 1),
             issued_over_useful=round(issued / hankel_flops["useful"], 2),
             dense_over_useful=round(hankel_flops["dense"] / hankel_flops["useful"], 1))
         ybytes = 4 if mode == "fp16" else 8
-        put(f"k_inv_dense<14,{mode}>", ndm * count * (N * ybytes + M * 8), sh["fft_inv"]["ms_per_launch"])
+        put(f"k_inv_dense<14,{mode}>", ndm * count * (N * ybytes + M * 8), sh["fft_inv"]["ms_per_launch"],
+            ckey=f"k_inv_dense_nb<14, {'true' if mode == 'fp16' else 'false'}>")
     if fold:
-        put("k_fold_phase", fold["alg_bytes_per_launch"], fold["ms_per_launch"])
+        put("k_fold_phase", fold["alg_bytes_per_launch"], fold["ms_per_launch"], ckey="k_fold_phase<")
     return out
 
 
