@@ -380,12 +380,13 @@ def draw_pulsars(n_examples, dm_max, period_range, duty_range, snr_range, seed):
 
 
 def make_dataset(pulsars, n_samples, bw, f0, frontend, judge, label_halfwidth=8, label_smooth=0, nbins=64,
-                 device="cuda:0", progress=None, label_stride=32):
+                 device="cuda:0", progress=None, label_stride=32, log=print):
     """Simulate each pulsar, run the front end, and label it with the plain
     coherent method: the fine trial with the highest folded S/N within
     +-label_halfwidth trials of the true DM, after a running mean over
     +-label_smooth trials (the S/N of neighbouring trials differs mostly by
-    noise when the pulse is wide compared with their smearing difference)."""
+    noise when the pulse is wide compared with their smearing difference).
+    With progress = k, log() receives a line every k pulsars."""
     import torch
     X = torch.empty((len(pulsars),) + frontend.shape, dtype=torch.float32, device=device)
     nf = len(judge.fine)
@@ -405,13 +406,14 @@ def make_dataset(pulsars, n_samples, bw, f0, frontend, judge, label_halfwidth=8,
         a, b = max(j0 - label_halfwidth, lo), min(j0 + label_halfwidth + 1, lo + width)
         labels.append(a + int(np.argmax(s[a - lo:b - lo])))
         if progress and (i + 1) % progress == 0:
-            print(f"  simulated and labelled {i + 1}/{len(pulsars)}", flush=True)
+            log(f"  simulated and labelled {i + 1}/{len(pulsars)}")
     return X, np.array(labels)
 
 
-def train(net, X, labels, epochs=10, batch=16, lr=2e-3, sigma=2.0, seed=0, cosine=False, progress=False):
+def train(net, X, labels, epochs=10, batch=16, lr=2e-3, sigma=2.0, seed=0, cosine=False, progress=False, log=print):
     """Adam on soft Gaussian targets (sigma in fine trials) around each
-    label; with cosine the learning rate decays to zero over the run."""
+    label; with cosine the learning rate decays to zero over the run.  With
+    progress, log() receives each epoch's mean loss."""
     import torch
     rng = np.random.default_rng(seed)
     n = X.shape[0]
@@ -432,7 +434,7 @@ def train(net, X, labels, epochs=10, batch=16, lr=2e-3, sigma=2.0, seed=0, cosin
             steps += 1
         losses.append(tot / max(1, steps))
         if progress:
-            print(f"  epoch {ep + 1}/{epochs}: loss {losses[-1]:.4f}", flush=True)
+            log(f"  epoch {ep + 1}/{epochs}: loss {losses[-1]:.4f}")
     return losses
 
 

@@ -126,7 +126,7 @@ def test_fold_and_harmonics_match_oracle(device):
 
 
 @pytest.mark.timeout(900)
-def test_trained_network_gain(device):
+def test_trained_network_gain(device, capsys):
     """The stated configuration: 2^26-sample baseband at 64 MHz (f0 1368 MHz),
     a 256-channel power spectrogram at 64 us (16 averaged 256-point spectra
     per bin), 20,000 simulated training pulsars with DM uniform on [0, 500],
@@ -145,6 +145,13 @@ def test_trained_network_gain(device):
     import time
 
     import torch
+
+    def say(msg):
+        # progress reaches the terminal while the test runs (a run of several
+        # minutes otherwise prints nothing until it ends)
+        with capsys.disabled():
+            print(msg, flush=True)
+
     bw, f0, n = 64e6, 1368e6, 1 << 26
     dm_max, n_fine, k = 500.0, 1024, 16
     fine = np.linspace(0, dm_max, n_fine).astype(np.float32)
@@ -154,13 +161,13 @@ def test_trained_network_gain(device):
     draw = dict(period_range=(0.001, 1.0), duty_range=(0.01, 0.10), snr_range=(5.0, 30.0))
     ptr = sorted(dmnet.draw_pulsars(20000, dm_max, seed=101, **draw), key=lambda m: m["dm"])
     pte = dmnet.draw_pulsars(64, dm_max, seed=202, **draw)
-    print(f"front end: {fe.T} spectrogram bins of {fe.tbin * 1e6:.0f} us, network input {fe.shape}; label skip "
-          f"{labeller.skip} samples", flush=True)
+    say(f"front end: {fe.T} spectrogram bins of {fe.tbin * 1e6:.0f} us, network input {fe.shape}; label skip "
+          f"{labeller.skip} samples")
     Xtr, ltr = dmnet.make_dataset(ptr, n, bw, f0, fe, labeller, device=device, progress=2000, label_halfwidth=3,
-                                  label_stride=4)
+                                  label_stride=4, log=say)
     labeller.close()
-    print(f"simulated and labelled 20000 pulsars in {time.time() - t0:.0f} s "
-          f"({labeller.trials_run} coherent trials)", flush=True)
+    say(f"simulated and labelled 20000 pulsars in {time.time() - t0:.0f} s "
+          f"({labeller.trials_run} coherent trials)")
     judge = dmnet.Judge(bw, f0, fine, n, count=256, device=device, max_engines=4)
     Xte = torch.empty((len(pte),) + fe.shape, dtype=torch.float32, device=device)
     fulls = []
@@ -170,13 +177,14 @@ def test_trained_network_gain(device):
         fe(u, Xte[i], p["period"])
         fulls.append(judge.snr_full(u, p["period"], chunk=256))
     judge.close()
-    print(f"held-out front ends and full-grid judges done at {time.time() - t0:.0f} s", flush=True)
+    say(f"held-out front ends and full-grid judges done at {time.time() - t0:.0f} s")
     net = dmnet.DMNet(c_in=fe.shape[0], n_out=n_fine, width=96, kw=9, dils=(1, 2, 4), seed=0, device=device,
                       head="conv")
     assert 150_000 < net.n_params < 250_000
     untrained = net.forward(Xte).cpu().numpy()
-    losses = dmnet.train(net, Xtr, ltr, epochs=10, batch=64, lr=1e-3, sigma=2.0, cosine=True, progress=True)
-    print(f"trained at {time.time() - t0:.0f} s", flush=True)
+    losses = dmnet.train(net, Xtr, ltr, epochs=10, batch=64, lr=1e-3, sigma=2.0, cosine=True, progress=True,
+                         log=say)
+    say(f"trained at {time.time() - t0:.0f} s")
     # soft targets (sigma 2 trials) and many pulsars too weak to locate keep
     # the loss well above the targets' entropy; the gain is judged below
     assert losses[-1] < losses[0] - 0.1, losses
@@ -199,10 +207,10 @@ def test_trained_network_gain(device):
     rec = {key: np.array(v) for key, v in rec.items()}
     det = np.array(best_all) >= 8.0   # pulsars the full grid detects
     for key, v in rec.items():
-        print(f"{key:17s}: recovered S/N fraction median {np.median(v):.3f} (detected: {np.median(v[det]):.3f}), "
+        say(f"{key:17s}: recovered S/N fraction median {np.median(v):.3f} (detected: {np.median(v[det]):.3f}), "
               f">=0.8 in {np.mean(v >= 0.8):.2f} of {len(v)} ({np.mean(v[det] >= 0.8):.2f} of {det.sum()} "
               f"detected)")
-    print(f"trials per pulsar with fallback: mean {np.mean(trials):.1f} of {n_fine} "
+    say(f"trials per pulsar with fallback: mean {np.mean(trials):.1f} of {n_fine} "
           f"({n_fine / np.mean(trials):.1f}x fewer than the full grid)")
     assert det.sum() >= 32
     assert np.mean(rec["network"][det] >= 0.8) >= 0.7
