@@ -82,6 +82,14 @@ def test_delay_invariants_and_brute_force(bw, dm, f0, M):
     pos = {int(np.nonzero(order == m)[0][0]) for m in mism}
     for p in pos:
         assert any(abs(p - b) <= 1 for b in bpos), (p, len(mism))
+    # ... and only on a bin whose exact segment floor(tau*fs/M) differs from
+    # a frequency neighbour's (the bin next to where tau*fs/M crosses an
+    # integer), off by exactly one segment
+    bo = brute[order]
+    for p in pos:
+        nb = [bo[q] for q in (p - 1, p + 1) if 0 <= q < N]
+        assert any(v != bo[p] for v in nb), p
+    assert np.all(np.abs(brute[mism] - d[mism]) == 1)
 
 
 def test_dm_zero_is_single_segment():
@@ -335,3 +343,36 @@ def test_fold_call_split_invariance():
     for k in range(10):
         b.fold_data(x[k * 6000:(k + 1) * 6000])
     np.testing.assert_array_equal(a.raw(), b.raw())
+
+
+@pytest.mark.parametrize("nd,skipped", [(1, 1), (999, 1), (1000, 1), (1001, 2), (2500, 3), (3000, 3), (3001, 4)])
+def test_fold_discard_count_closed_form(nd, skipped):
+    # fold_gpu.cpp:29-33: discard_samples(Nd) skips the first ceil(Nd / P)
+    # whole periods.  P*fs = 1000 exactly (no drift); period k holds the
+    # value k+1 everywhere, so every raw bin sums the periods kept.
+    P, nper = 1000, 10
+    x = np.repeat(np.arange(1, nper + 1, dtype=np.float32), P)
+    f = oracle.Fold(1.0, 1000.0, x.size, -1)
+    f.discard_samples(nd)
+    f.fold_data(x)
+    want = float(sum(range(skipped + 1, nper + 1)))
+    raw = f.raw()
+    assert raw.shape == (P,) and np.all(raw == want), (raw[:3], want)
+
+
+@pytest.mark.parametrize("time_bin,per_bin", [(100, 10), (250, 4), (300, 3), (999, 1)])
+def test_fold_bins_average_closed_form(time_bin, per_bin):
+    # fold_gpu.cpp:175-194: bins_per_time_bin = P / time_bin (integer
+    # division); output bin i is the mean of raw bins [i*b, (i+1)*b), so only
+    # the first time_bin*b raw bins are used.  A ramp x[n] = n mod P folded
+    # over nper periods gives raw[j] = nper*j and bin i = nper*(b*i + (b-1)/2).
+    P, nper = 1000, 6
+    x = np.tile(np.arange(P, dtype=np.float32), nper)
+    f = oracle.Fold(1.0, 1000.0, x.size, time_bin)
+    f.discard_samples(0)
+    f.fold_data(x)
+    f.bins()
+    np.testing.assert_array_equal(f.raw(), nper * np.arange(P, dtype=np.float32))
+    i = np.arange(time_bin)
+    want = (nper * (per_bin * i + (per_bin - 1) / 2.0)).astype(np.float32)
+    np.testing.assert_array_equal(f.folded(), want)
