@@ -70,6 +70,9 @@ def parse():
     ap.add_argument("--cpu-log2-samples", type=int, default=23)
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the short runs of the other workloads (single trial at 16 and 128 MHz, 24 streams)")
+    ap.add_argument("--only-config", choices=("single16", "single128", "streams24"),
+                    help="run only this one of the other workloads and print its results as one JSON line (no "
+                         "headline; for profiling a workload on its own)")
     return ap.parse_args()
 
 
@@ -95,6 +98,9 @@ def main():
     dms = np.linspace(0.0, 100.0, args.num_dms).astype(np.float32)
     if args.pmc_workload:
         pmc_workload(ms, torch, dev, bw, f0, dms, args)
+        return
+    if args.only_config:
+        print(json.dumps({"other_configs": other_configs(ms, torch, dev, only=args.only_config)}), flush=True)
         return
     t0 = time.time()
     u = simulate.pulsar_u16(L, bw, f0, 57.3, 0.008, duty=0.05, snr=0.1, seed=0x5EED0003)
@@ -321,13 +327,25 @@ def osm_rows(ms, bw, dm, f0, d_in, forms):
         r["rel_l2_vs_fp64_osm"] = dict(form=rel(rows[0], seg), msosm_f64=rel(ref[0], seg), osm_fftpoint=2 * Mo)
 
 
-def other_configs(ms, torch, dev):
+def other_configs(ms, torch, dev, only=None):
     """Short runs of the other workloads (inputs resident in HBM, uniform
     random codes): a single trial at 16 MHz (DM 10, FFT 1024) and at 128 MHz
     (DM 100 at its plan, FFT 2^16 with K 152, and at FFT 2^14 with K 608), and
     24 independent streams of 8-64 MHz, 2^20-2^25 samples, 1-64 trials,
     autotuned FFT sizes, forms and launch shapes, up to 2^22 samples per call,
-    each phase-folded, replayed as one graph."""
+    each phase-folded, replayed as one graph.  only = "single16",
+    "single128" or "streams24" runs that one alone."""
+    out = {}
+    if only in (None, "single16"):
+        out.update(config_single16(ms, torch, dev))
+    if only in (None, "single128"):
+        out.update(config_single128(ms, torch, dev))
+    if only in (None, "streams24"):
+        out.update(config_streams24(ms, torch, dev))
+    return out
+
+
+def config_single16(ms, torch, dev):
     out = {}
     L = 1 << 22
     d_in = random_u16_words(torch, dev, L, 0x5EED0001)
@@ -338,7 +356,11 @@ def other_configs(ms, torch, dev):
                                L, steps=3, gate=gate)
     osm_rows(ms, 16e6, 10.0, 1392e6, d_in, c1)
     out["single trial, 16 MHz, DM 10, 2^22 samples, FFT 1024, 1024 blocks per call"] = c1
-    del d_in
+    return out
+
+
+def config_single128(ms, torch, dev):
+    out = {}
     L = 1 << 28
     d_in = random_u16_words(torch, dev, L, 0x5EED0002)
     c2 = {"plain_f32, FFT 2^16 (K 152)": time_engine(ms, torch, ms.Plan(128e6, 100.0, 1336e6, fftpoint=65536,
@@ -355,6 +377,11 @@ def other_configs(ms, torch, dev):
     out["single trial, 128 MHz, DM 100, 2^28 samples, 2^23 samples per call"] = c2
     del d_in
     torch.cuda.empty_cache()
+    return out
+
+
+def config_streams24(ms, torch, dev):
+    out = {}
     rng = np.random.Generator(np.random.Philox(0x5EED0004))
     engines, inputs, lens, periods, shapes, work = [], [], [], [], [], 0
     for s in range(24):
