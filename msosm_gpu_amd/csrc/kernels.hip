@@ -432,6 +432,83 @@ __global__ __launch_bounds__(fft_threads(LOG2N)) void k_fwd_u16(
     }
 }
 
+// Fused forward for calls with many 2^14-point windows (more than the GPU
+// runs at once): each workgroup transforms a contiguous run of nwpw windows,
+// so the twiddle tables are built once per run, the next window's codes are
+// loaded into registers while the current window is transformed, and the
+// half of each window that the previous one also read is an L2 hit on the
+// same XCD.  The middle pass reads its twiddles from the row table.  Same
+// outputs as k_fwd_u16 (ring slot (g0 + w) mod rx, FP16 scaled by 1/N or
+// FP32; the last window's second half to tail_out).
+template <bool F16>
+__global__ __launch_bounds__((1 << 14) / 32) void k_fwd_u16_nb(
+    const uint32_t *__restrict__ cur, const uint32_t *__restrict__ tail, uint32_t *__restrict__ tail_out,
+    void *__restrict__ ring, int count, long long g0, int rx, int nwpw)
+{
+    constexpr int LOG2N = 14, E = 32;
+    using S = FftShape<LOG2N, E>;
+    constexpr long long N = S::N, M = S::N / 2;
+    constexpr int NT = S::NT;
+    __shared__ float2 smem[S::NPAD + S::TWLO + S::TWHI + kTwRowElems];
+    float2 *twlo = smem + S::NPAD, *twhi = twlo + S::TWLO, *twrows = twhi + S::TWHI;
+    fill_twiddles<LOG2N, E>(twlo, twhi, threadIdx.x, blockDim.x);
+    fill_twrows(twrows, threadIdx.x, blockDim.x);
+    const int t = threadIdx.x;
+    const int w0 = blockIdx.x * nwpw;
+    const int nw = min(nwpw, count - w0);
+    if (nw <= 0)
+        return; // uniform: the whole workgroup leaves
+    uint32_t u[E];
+    // window w covers input samples [(w - 1) M, (w + 1) M) of this call; the
+    // first M of window 0 are the previous call's last M (tail).  Only the
+    // run's first load can start at window 0.
+#pragma unroll
+    for (int m = 0; m < E; m++) {
+        const long long rel = (long long)(w0 - 1) * M + t + m * NT;
+        u[m] = *(rel < 0 ? tail + (M + rel) : cur + rel);
+    }
+    // Wait for the first window here: the loop's first iteration then has
+    // nothing in flight, and the compiler's waits at the top of the loop
+    // follow the steady state alone (vector-memory operations retire in
+    // issue order; the next window's loads are issued before this window's
+    // ring stores, so they are waited for without the stores).  The loop
+    // body has no branches for the same reason; the last window of a run
+    // loads itself again, which leaves the call's last window in u for the
+    // tail.
+    __builtin_amdgcn_s_waitcnt(0x0F70); // vmcnt(0)
+    for (int i = 0; i < nw; i++) {
+        const int w = w0 + i;
+        float2 v[E];
+#pragma unroll
+        for (int m = 0; m < E; m++)
+            v[m] = unpack_u16pair(u[m]);
+        const uint32_t *next = cur + (long long)min(w, count - 2) * M + t; // window min(w + 1, count - 1)
+#pragma unroll
+        for (int m = 0; m < E; m++)
+            u[m] = next[m * NT];
+        fft_block<LOG2N, E, false, true>(v, smem, twlo, twhi, t, twrows);
+        const long long slot = (g0 + w) % rx;
+        if constexpr (F16) {
+            __half2 *dst = reinterpret_cast<__half2 *>(ring) + slot * N;
+            constexpr float invN = 1.0f / (float)N;
+#pragma unroll
+            for (int m = 0; m < E; m++)
+                dst[t + m * NT] = __floats2half2_rn(v[m].x * invN, v[m].y * invN);
+        } else {
+            float2 *dst = reinterpret_cast<float2 *>(ring) + slot * N;
+#pragma unroll
+            for (int m = 0; m < E; m++)
+                dst[t + m * NT] = v[m];
+        }
+    }
+    // the call's last window's second half is the next call's history
+    if (w0 + nw == count) {
+#pragma unroll
+        for (int m = E / 2; m < E; m++)
+            tail_out[t + m * NT - M] = u[m];
+    }
+}
+
 // Fused inverse: unit u = (dm, block b) with blocks of one DM consecutive.
 // Y[n] = X_{g-d(n)}[n] * H[n] (zero before the stream start), inverse FFT,
 // keep points [M, 2M), scale: FP32 ring/H -> 1/N^2 (the reference's two 1/N
@@ -711,6 +788,21 @@ hipError_t launch_fft_c2c(const float2 *in, float2 *out, int log2n, long long ba
 
 bool fused_supported(int log2n) { return log2n >= 2 && log2n <= kMaxLog2SinglePass; }
 
+// Windows per workgroup of the 2^14-point forward transform: one while the
+// call's windows fit in one round of workgroups (one such workgroup per CU:
+// its LDS), else runs that spread the windows over the CUs once.
+static int fwd_windows_per_wg(int count)
+{
+    static const int cus = [] {
+        int dev = 0, n = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0)
+            n = 256;
+        return n;
+    }();
+    return count <= cus ? 1 : (count + cus - 1) / cus;
+}
+
 template <int LOG2N>
 static hipError_t fwd_u16(const uint32_t *cur, const uint32_t *tail, uint32_t *tail_out, void *ring, bool f16,
                           int count, long long g0, int rx, bool mfma_order, hipStream_t s)
@@ -729,6 +821,21 @@ static hipError_t fwd_u16(const uint32_t *cur, const uint32_t *tail, uint32_t *t
         }
     } else if (mfma_order) {
         return hipErrorInvalidValue;
+    }
+    if constexpr (LOG2N == 14) {
+        // more windows than run at once: runs of windows per workgroup
+        const int nwpw = fwd_windows_per_wg(count);
+        if (nwpw > 1) {
+            const int g = (count + nwpw - 1) / nwpw;
+            if (f16)
+                hipLaunchKernelGGL((k_fwd_u16_nb<true>), dim3(g), dim3(threads), 0, s, cur, tail, tail_out, ring,
+                                   count, g0, rx, nwpw);
+            else
+                hipLaunchKernelGGL((k_fwd_u16_nb<false>), dim3(g), dim3(threads), 0, s, cur, tail, tail_out, ring,
+                                   count, g0, rx, nwpw);
+            MSOSM_LAUNCH_CHECK();
+            return hipSuccess;
+        }
     }
     if (f16)
         hipLaunchKernelGGL((k_fwd_u16<LOG2N, true>), dim3(grid), dim3(threads), 0, s, cur, tail, tail_out, ring, count,

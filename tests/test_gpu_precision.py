@@ -159,3 +159,28 @@ def test_error_against_single_segment_osm(device):
     one("64 MHz multi-trial", 64e6, 1368e6, [14.5, 57.3, 100.0], 8192, 64, 1 << 24, 2 * (1 << 22) // 8192 + 8, 2,
         ("plain_f32", "mixed_f16", "mixed_f16_mfma", "hankel_f16"), 0x5EED0003, 57.3, 0.008)
     print("PRECISION_REPORT " + json.dumps(rows), flush=True)
+
+
+@pytest.mark.parametrize("form,gate", [(ms.FORM_FUSED_F16, 1e-3), (ms.FORM_FUSED_F32, 1e-5)])
+@pytest.mark.parametrize("count,calls", [(300, 3), (1024, 2)])
+def test_calls_of_many_windows_vs_fp64(device, form, gate, count, calls):
+    """Calls that hold more 2^14-point windows than the GPU runs at once
+    (128 MHz, DM 100, K 608): the forward transform then runs windows in
+    runs per workgroup (300: runs of 2; 1024: runs of 4) and hands the last
+    window's history to the next call.  The last call's last two blocks
+    against the FP64 same-plan oracle, past the 608-block delay line."""
+    import torch
+    bw, dm, f0, M = 128e6, 100.0, 1336e6, 8192
+    p = ms.Plan(bw, dm, f0, fftpoint=2 * M, count=count, form=form)
+    assert p.M == M and p.K_max == 608
+    n = calls * count * M
+    rng = np.random.Generator(np.random.Philox(0x5EED0070 + count))
+    u = rng.integers(1, 65536, size=2 * n, dtype=np.uint16)
+    d_in = torch.from_numpy(u.view(np.int32).copy()).to(device)
+    print(f"many-window calls form={form} count={count} calls={calls}", flush=True)
+    got = _run_blocks(p, d_in, calls, (count - 2, count))[0]
+    b0 = (calls - 1) * count + count - 2
+    ref = _same_plan_fp64(u, bw, f0, [dm], M, b0, 2)[0]
+    err = rel_l2(got, ref)
+    print(f"  rel L2 vs FP64 {err:.3e} (gate {gate:g})", flush=True)
+    assert err < gate
