@@ -439,7 +439,7 @@ __global__ __launch_bounds__(fft_threads(LOG2N)) void k_fwd_u16(
 // half of each window that the previous one also read is an L2 hit on the
 // same XCD.  The middle pass reads its twiddles from the row table.  Same
 // outputs as k_fwd_u16 (ring slot (g0 + w) mod rx, FP16 scaled by 1/N or
-// FP32; the last window's second half to tail_out).
+// FP32; the last window's second half to tail_out).  Needs count >= 2.
 template <bool F16>
 __global__ __launch_bounds__((1 << 14) / 32) void k_fwd_u16_nb(
     const uint32_t *__restrict__ cur, const uint32_t *__restrict__ tail, uint32_t *__restrict__ tail_out,
@@ -482,7 +482,10 @@ __global__ __launch_bounds__((1 << 14) / 32) void k_fwd_u16_nb(
 #pragma unroll
         for (int m = 0; m < E; m++)
             v[m] = unpack_u16pair(u[m]);
-        const uint32_t *next = cur + (long long)min(w, count - 2) * M + t; // window min(w + 1, count - 1)
+        // window min(w + 1, count - 1) starts at (min(w + 1, count - 1) - 1) M;
+        // with count >= 2 that is in [0, (count - 2) M] and the window ends by
+        // count M, inside this call's input
+        const uint32_t *next = cur + (long long)min(w, count - 2) * M + t;
 #pragma unroll
         for (int m = 0; m < E; m++)
             u[m] = next[m * NT];
@@ -823,9 +826,10 @@ static hipError_t fwd_u16(const uint32_t *cur, const uint32_t *tail, uint32_t *t
         return hipErrorInvalidValue;
     }
     if constexpr (LOG2N == 14) {
-        // more windows than run at once: runs of windows per workgroup
+        // more windows than run at once: runs of windows per workgroup (the
+        // kernel reloads the call's last window, which needs count >= 2)
         const int nwpw = fwd_windows_per_wg(count);
-        if (nwpw > 1) {
+        if (nwpw > 1 && count >= 2) {
             const int g = (count + nwpw - 1) / nwpw;
             if (f16)
                 hipLaunchKernelGGL((k_fwd_u16_nb<true>), dim3(g), dim3(threads), 0, s, cur, tail, tail_out, ring,
