@@ -22,10 +22,22 @@
 //             window (FP16 scaled by 1/N, or FP32)
 //   inv_rows  (unit, k1), over a chunk of units: gather * H (FP32 products),
 //             inverse DFT_N2, twiddle W_N^(-k1 m2), store U[unit][k1][m2]
+//             (FP32; in the FP16 form FP16 scaled by 2^-7, see below)
 //   inv_cols  (unit, tile of columns m2): inverse DFT_N1 of column m2, keep
 //             m1 >= N1/2, scale, streaming store to out[trial][block][.]
-// The inverse runs chunk by chunk so its FP32 scratch (about 128 MiB) is
+// The inverse runs chunk by chunk so its scratch (about 128 MiB in FP32) is
 // re-used while it is still in the last-level cache.
+//
+// FP16 row scratch: a row of U is an inverse DFT of N2 products Z = (X/N)
+// (H/(2M)).  By Parseval sum |X/N|^2 <= max|x|^2 <= 46341^2 over all N bins,
+// so over the N2 bins of one row sum |X/N| <= sqrt(N2) * 46341 (Cauchy-Schwarz)
+// and |U| <= 128 * 46341 * max|H/(2M)|.  H[2k] = M C[k] with |C| = 1, and an
+// odd bin is the zero-padded interpolation sum_k C[k] 2 / (1 - W_2M^(j-2k)),
+// so |H/(2M)| <= ln(M)/pi + 0.4 (4.8 at M = 2^20; even bins 1/2, the plans
+// measured 0.7-1.5 from M = 2^9 to 2^20).  Stored times 2^-9 (exact), |U|
+// stays below 65504 for any uint16 input and any plan here; the column pass
+// multiplies 2^9 back into its output scale.  Typical values (noise inputs)
+// sit near 0.4 after the scale, far above FP16's normal minimum.
 #include <hip/hip_runtime.h>
 #include <hip/hip_fp16.h>
 
@@ -46,6 +58,7 @@ constexpr int kColThreads = 256;
 // windows (units) per forward (inverse) launch pair: about this many
 // scratch bytes by default
 constexpr size_t kScratchBytes = size_t(128) << 20;
+constexpr float kRowScratchScale = 1.0f / 512.0f; // FP16 row scratch of the inverse (below)
 
 template <int L1>
 struct ColShape {
@@ -186,13 +199,23 @@ __global__ __launch_bounds__(RowS::NT) void k_big_inv_rows(const void *__restric
         v[m] = R::mul(rg[(long long)sl * N + p0 + q], hrow[q]);
     }
     fft_block<kRowLog2, E, true, true>(v, smem, twlo, twhi, t, twrows); // barriers: step[] is visible after it
-    float2 *dst = U + ((long long)ul * N1 + k1) * kN2;
+    const long long roff = ((long long)ul * N1 + k1) * kN2;
+    if constexpr (F16) {
+        __half2 *dst = reinterpret_cast<__half2 *>(U) + roff;
 #pragma unroll
-    for (int m = 0; m < E; m++)
-        dst[t + m * NT] = cmul(v[m], cmul(base, step[m]));
+        for (int m = 0; m < E; m++) {
+            const float2 r = cmul(v[m], cmul(base, step[m]));
+            dst[t + m * NT] = __floats2half2_rn(r.x * kRowScratchScale, r.y * kRowScratchScale);
+        }
+    } else {
+        float2 *dst = U + roff;
+#pragma unroll
+        for (int m = 0; m < E; m++)
+            dst[t + m * NT] = cmul(v[m], cmul(base, step[m]));
+    }
 }
 
-template <int L1>
+template <int L1, bool F16>
 __global__ __launch_bounds__(kColThreads) void k_big_inv_cols(const float2 *__restrict__ U, float2 *__restrict__ out,
                                                               int u0, float scale)
 {
@@ -208,11 +231,18 @@ __global__ __launch_bounds__(kColThreads) void k_big_inv_cols(const float2 *__re
     constexpr int tiles = (int)(kN2 / C);
     const int ul = blockIdx.x / tiles;
     const long long m2 = (long long)(blockIdx.x % tiles) * C + c;
-    const float2 *src = U + (long long)ul * N + m2;
     float2 v[E];
+    if constexpr (F16) {
+        const __half2 *src = reinterpret_cast<const __half2 *>(U) + (long long)ul * N + m2;
 #pragma unroll
-    for (int m = 0; m < E; m++)
-        v[m] = src[(long long)(t + m * NT) * kN2];
+        for (int m = 0; m < E; m++)
+            v[m] = __half22float2(src[(long long)(t + m * NT) * kN2]);
+    } else {
+        const float2 *src = U + (long long)ul * N + m2;
+#pragma unroll
+        for (int m = 0; m < E; m++)
+            v[m] = src[(long long)(t + m * NT) * kN2];
+    }
     fft_block<L1, E, true>(v, smem + c * S::NPAD, twlo, twhi, t);
     // unit u = trial * count + block: its output row starts at u * M
     float2 *dst = out + (long long)(u0 + ul) * M + m2;
@@ -327,7 +357,8 @@ hipError_t launch_big_inverse(const void *ring, const void *Hp, const uint16_t *
     const int l1 = log2n - kRowLog2;
     const long long N = 1ll << log2n;
     const long long units = (long long)ndm * count;
-    const float scale = f16 ? 1.0f : 1.0f / ((float)N * (float)N);
+    // FP16: the products carry the whole scale, the row scratch 2^-9 of it
+    const float scale = f16 ? 1.0f / kRowScratchScale : 1.0f / ((float)N * (float)N);
     for (long long u0 = 0; u0 < units; u0 += chunk) {
         const int nu = (int)std::min<long long>(chunk, units - u0);
         const unsigned grid_r = (unsigned)((long long)nu << l1);
@@ -341,8 +372,12 @@ hipError_t launch_big_inverse(const void *ring, const void *Hp, const uint16_t *
         if (e != hipSuccess)
             return e;
 #define CALL_I(L)                                                                                                \
-    hipLaunchKernelGGL((k_big_inv_cols<L>), dim3((unsigned)((long long)nu * (kN2 / ColShape<L>::C))),             \
-                       dim3(kColThreads), 0, s, scratch, out, (int)u0, scale)
+    if (f16)                                                                                                     \
+        hipLaunchKernelGGL((k_big_inv_cols<L, true>), dim3((unsigned)((long long)nu * (kN2 / ColShape<L>::C))),   \
+                           dim3(kColThreads), 0, s, scratch, out, (int)u0, scale);                               \
+    else                                                                                                         \
+        hipLaunchKernelGGL((k_big_inv_cols<L, false>), dim3((unsigned)((long long)nu * (kN2 / ColShape<L>::C))),  \
+                           dim3(kColThreads), 0, s, scratch, out, (int)u0, scale)
         MSOSM_BIG_DISPATCH(l1, CALL_I)
 #undef CALL_I
         e = hipGetLastError();
