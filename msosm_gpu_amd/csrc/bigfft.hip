@@ -160,6 +160,64 @@ __global__ __launch_bounds__(RowS::NT) void k_big_fwd_rows(const void *__restric
     }
 }
 
+// k_big_fwd_rows for calls with more rows than CUs: each workgroup transforms
+// a run of nrpw consecutive rows r = wl N1 + k1 of the column scratch (row r
+// starts at r N2), builds its twiddle tables once, and loads the next row
+// while the current one is transformed (issued before the current row's ring
+// stores; the loop body has no branches, so the compiler's waits at its top
+// wait for those loads only).  The run's last row loads itself again.
+template <bool F16>
+__global__ __launch_bounds__(RowS::NT) void k_big_fwd_rows_nb(const void *__restrict__ T, void *__restrict__ ring,
+                                                              int w0, int l1, long long g0, int rx, int nrows,
+                                                              int nrpw)
+{
+    using TT = typename RingElem<F16>::T;
+    constexpr int E = kRowE, NT = RowS::NT;
+    __shared__ float2 smem[RowS::NPAD + RowS::TWLO + RowS::TWHI + kTwRowElems];
+    float2 *twlo = smem + RowS::NPAD, *twhi = twlo + RowS::TWLO, *twrows = twhi + RowS::TWHI;
+    fill_twiddles<kRowLog2, E>(twlo, twhi, threadIdx.x, blockDim.x);
+    fill_twrows(twrows, threadIdx.x, blockDim.x);
+    const int t = threadIdx.x;
+    const long long N1 = 1ll << l1, N = N1 * kN2;
+    const int r0 = blockIdx.x * nrpw;
+    const int nr = min(nrpw, nrows - r0);
+    if (nr <= 0)
+        return; // uniform: the whole workgroup leaves
+    const TT *src = reinterpret_cast<const TT *>(T);
+    TT raw[E];
+#pragma unroll
+    for (int m = 0; m < E; m++)
+        raw[m] = src[(long long)r0 * kN2 + t + m * NT];
+    __builtin_amdgcn_s_waitcnt(0x0F70); // vmcnt(0): the loop's first iteration has nothing in flight
+    for (int i = 0; i < nr; i++) {
+        const int r = r0 + i;
+        float2 v[E];
+#pragma unroll
+        for (int m = 0; m < E; m++)
+            v[m] = RingElem<F16>::f2(raw[m]);
+        const TT *next = src + (long long)min(r + 1, nrows - 1) * kN2 + t;
+#pragma unroll
+        for (int m = 0; m < E; m++)
+            raw[m] = next[m * NT];
+        fft_block<kRowLog2, E, false, true>(v, smem, twlo, twhi, t, twrows);
+        const int w = w0 + (r >> l1), k1 = r & (int)(N1 - 1);
+        const long long slot = (g0 + w) % rx;
+        if constexpr (F16) {
+            // the scratch carried 1/N1: the ring's X/N is this row's output / N2
+            __half2 *dst = reinterpret_cast<__half2 *>(ring) + slot * N + (long long)k1 * kN2;
+            const float invN = 1.0f / (float)kN2;
+#pragma unroll
+            for (int m = 0; m < E; m++)
+                dst[t + m * NT] = __floats2half2_rn(v[m].x * invN, v[m].y * invN);
+        } else {
+            float2 *dst = reinterpret_cast<float2 *>(ring) + slot * N + (long long)k1 * kN2;
+#pragma unroll
+            for (int m = 0; m < E; m++)
+                dst[t + m * NT] = v[m];
+        }
+    }
+}
+
 // A block before the stream start gathers from slot rx + g - d, a slot no
 // call has written yet (blocks newer than the call's last): the ring is
 // zeroed at creation and reset, so those reads are zero.
@@ -285,6 +343,21 @@ __global__ void k_big_permute(const float2 *__restrict__ src, void *__restrict__
 
 } // namespace
 
+// Rows per workgroup of the 2^14-point row transforms: one while a launch's
+// rows fit in one round of workgroups (one such workgroup per CU: its LDS),
+// else runs that spread the rows over the CUs once.
+static int rows_per_wg(int nrows)
+{
+    static const int cus = [] {
+        int dev = 0, n = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0)
+            n = 256;
+        return n;
+    }();
+    return nrows <= cus ? 1 : (nrows + cus - 1) / cus;
+}
+
 bool big_supported(int log2n) { return log2n > kRowLog2 && log2n <= kRowLog2 + 7; }
 
 long long big_position_bin(int log2n, long long p)
@@ -334,13 +407,25 @@ hipError_t launch_big_forward(const uint32_t *cur, const uint32_t *tail, uint32_
         hipError_t e = hipGetLastError();
         if (e != hipSuccess)
             return e;
-        const unsigned grid = (unsigned)((long long)nw << l1);
-        if (f16)
-            hipLaunchKernelGGL((k_big_fwd_rows<true>), dim3(grid), dim3(RowS::NT), 0, s, scratch, ring, w0, l1, g0,
-                               rx);
-        else
-            hipLaunchKernelGGL((k_big_fwd_rows<false>), dim3(grid), dim3(RowS::NT), 0, s, scratch, ring, w0, l1, g0,
-                               rx);
+        const int nrows = nw << l1;
+        const int nrpw = rows_per_wg(nrows);
+        if (nrpw > 1) {
+            const unsigned grid = (unsigned)((nrows + nrpw - 1) / nrpw);
+            if (f16)
+                hipLaunchKernelGGL((k_big_fwd_rows_nb<true>), dim3(grid), dim3(RowS::NT), 0, s, scratch, ring, w0,
+                                   l1, g0, rx, nrows, nrpw);
+            else
+                hipLaunchKernelGGL((k_big_fwd_rows_nb<false>), dim3(grid), dim3(RowS::NT), 0, s, scratch, ring, w0,
+                                   l1, g0, rx, nrows, nrpw);
+        } else {
+            const unsigned grid = (unsigned)nrows;
+            if (f16)
+                hipLaunchKernelGGL((k_big_fwd_rows<true>), dim3(grid), dim3(RowS::NT), 0, s, scratch, ring, w0, l1,
+                                   g0, rx);
+            else
+                hipLaunchKernelGGL((k_big_fwd_rows<false>), dim3(grid), dim3(RowS::NT), 0, s, scratch, ring, w0, l1,
+                                   g0, rx);
+        }
         e = hipGetLastError();
         if (e != hipSuccess)
             return e;

@@ -105,3 +105,30 @@ def test_fft_plan_matches_entry_point(device, log2n, batch):
         torch.cuda.synchronize()
         np.testing.assert_array_equal(a.cpu().numpy(), b.cpu().numpy())
     plan.close()
+
+
+@pytest.mark.parametrize("kind", ["full_scale_noise", "pulse_train_32", "constant"])
+def test_fp16_row_scratch_full_scale(device, kind):
+    """The FP16-storage form above 2^14 keeps the inverse's row scratch in
+    FP16 scaled by 2^-9: inputs at the uint16 limits (maximum power spread
+    over all bins, a pulse train whose spectrum adds coherently within rows,
+    a constant whose whole power sits in one bin) stay finite and within the
+    mixed-precision gate against FP64, at FFT 2^16 (4 rows of 2^14)."""
+    M, count, calls = 1 << 15, 3, 3
+    L = calls * count * M
+    rng = np.random.default_rng(12)
+    if kind == "full_scale_noise":
+        u = rng.choice(np.array([1, 65535], np.uint16), 2 * L)
+    elif kind == "pulse_train_32":
+        u = np.full(2 * L, 32768, np.uint16)
+        u[0::64] = 65535
+        u[1::64] = 65535
+    else:
+        u = np.full(2 * L, 65535, np.uint16)
+    bw, dm, f0 = 64e6, 20.0, 1368e6
+    p = ms.Plan(bw, dm, f0, fftpoint=2 * M, count=count, form=ms.FORM_FUSED_F16)
+    y = run_engine(p, u, device=device)[0]
+    assert np.all(np.isfinite(y))
+    err = rel_l2(y, oracle_out(bw, dm, f0, M, u))
+    print(f"fp16 row scratch {kind}: rel L2 vs FP64 {err:.2e}", flush=True)
+    assert err < 1e-3
