@@ -16,7 +16,7 @@
 //   fwd_cols  (window, 256-thread tile of columns n2): unpack the overlapped
 //             uint16 window (previous call's tail for negative offsets; the
 //             last window also writes this call's tail), DFT_N1, twiddle
-//             W_N^(n2 k1) (FP32 sincospi), store T[w][k1][n2] (FP32 scratch; in the
+//             W_N^(n2 k1) (ColumnTwiddles), store T[w][k1][n2] (FP32 scratch; in the
 //             FP16 form FP16 scaled by 1/N1, which keeps |T| <= 46341)
 //   fwd_rows  (window, k1): DFT_N2 of row k1, store to the ring slot of the
 //             window (FP16 scaled by 1/N, or FP32)
@@ -78,17 +78,6 @@ __device__ __forceinline__ float2 twiddle_f64(long long e, long long N, bool inv
     return make_float2((float)c, inv ? (float)s : (float)(-s));
 }
 
-// W_N^e (forward sign) from the FP32 sincospi: 2 e / N (e < N <= 2^21, N a
-// power of two) is exact in FP32, and the result is within about an ulp of
-// the FP64 value rounded to FP32 -- as close as the row transforms' own
-// table-product twiddles -- at a fraction of the FP64 evaluation's cost.
-__device__ __forceinline__ float2 twiddle_f32(long long e, long long N)
-{
-    float s, c;
-    sincospif(2.0f * (float)e / (float)N, &s, &c);
-    return make_float2(c, -s);
-}
-
 template <int L1, bool F16>
 __global__ __launch_bounds__(kColThreads) void k_big_fwd_cols(const uint32_t *__restrict__ cur,
                                                               const uint32_t *__restrict__ tail,
@@ -136,32 +125,13 @@ __global__ __launch_bounds__(kColThreads) void k_big_fwd_cols(const uint32_t *__
     for (int m = 0; m < E; m++)
         v[m] = unpack_u16pair(u[m]);
     fft_block<L1, E, false>(v, smem + c * S::NPAD, twlo, twhi, t);
-    // Twiddles W_N^(n2 k1) for this thread's k1 = t + m NT: a s^m with
-    // a = W^(n2 t), s = W^(n2 NT), and s^m = s^(m mod 4) (s^4)^(m / 4) from
-    // s, s^4 and their products -- at most three sincospi per thread and a few
-    // roundings per twiddle (the FP32 forms stay within 1e-5 of FP64).
-    const float2 one = make_float2(1.f, 0.f);
-    const float2 tw_a = NT > 1 ? twiddle_f32((n2 * t) & (N - 1), N) : one;
-    const float2 s1 = twiddle_f32((n2 * NT) & (N - 1), N);
-    const float2 s2 = cmul(s1, s1), s3 = cmul(s2, s1);
-    float2 s4 = one, s8 = one, s12 = one;
-    if constexpr (E > 4) {
-        s4 = twiddle_f32((4 * n2 * NT) & (N - 1), N);
-        s8 = cmul(s4, s4);
-        s12 = cmul(s8, s4);
-    }
-    static_assert(E <= 16, "twiddle digits");
+    // W_N^(n2 k1) for this thread's k1 = t + m NT
+    const ColumnTwiddles<E, false> tw(NT > 1 ? n2 * t : 0, n2 * NT, N);
     const long long off = (long long)wl * N + n2;
 #pragma unroll
     for (int m = 0; m < E; m++) {
         const long long k1 = t + m * NT;
-        const int lo = m & 3, hi = m >> 2;
-        float2 tw = lo == 0 ? one : lo == 1 ? s1 : lo == 2 ? s2 : s3;
-        if (hi)
-            tw = lo == 0 ? (hi == 1 ? s4 : hi == 2 ? s8 : s12) : cmul(tw, hi == 1 ? s4 : hi == 2 ? s8 : s12);
-        if constexpr (NT > 1)
-            tw = cmul(tw_a, tw);
-        const float2 y = m == 0 && NT == 1 ? v[m] : cmul(v[m], tw);
+        const float2 y = (m == 0 && !tw.has_a) ? v[m] : cmul(v[m], tw(m));
         if constexpr (F16) {
             constexpr float s = 1.0f / (float)N1;
             reinterpret_cast<__half2 *>(T)[off + k1 * kN2] = __floats2half2_rn(y.x * s, y.y * s);

@@ -312,6 +312,64 @@ __device__ __forceinline__ void fft_pass(float2 (&v)[E], float2 *lds, const floa
     }
 }
 
+// Twiddles W_N^(e0 + e1 m) (forward: exp(-2 pi i e/N); inverse: conjugate)
+// for a thread's points m = 0 .. E-1, as used by the column passes of
+// four-step transforms (k1 = t + m NT: e0 = n2 t, e1 = n2 NT).  Evaluated from
+// at most four FP32 sincospi -- W^e0, W^e1, W^(4 e1), W^(16 e1); the exponents
+// are reduced mod N (a power of two <= 2^28, so 2 e / N is exact in FP32) --
+// and base-4 digit products of them: a few roundings per twiddle, against one
+// FP64 sincospi per point.
+template <int E, bool INV>
+struct ColumnTwiddles {
+    static_assert(E <= 64, "three base-4 digits");
+    float2 a, s1, s2, s3, q1, q2, q3, h1, h2, h3;
+    bool has_a;
+    __device__ __forceinline__ static float2 w(long long e, long long N)
+    {
+        float s, c;
+        sincospif(2.0f * (float)(e & (N - 1)) / (float)N, &s, &c);
+        return make_float2(c, INV ? s : -s);
+    }
+    __device__ __forceinline__ ColumnTwiddles(long long e0, long long e1, long long N)
+    {
+        const float2 one = make_float2(1.f, 0.f);
+        has_a = (e0 & (N - 1)) != 0;
+        a = has_a ? w(e0, N) : one;
+        s1 = w(e1, N);
+        s2 = cmul(s1, s1);
+        s3 = cmul(s2, s1);
+        q1 = q2 = q3 = h1 = h2 = h3 = one;
+        if constexpr (E > 4) {
+            q1 = w(4 * e1, N);
+            q2 = cmul(q1, q1);
+            q3 = cmul(q2, q1);
+        }
+        if constexpr (E > 16) {
+            h1 = w(16 * e1, N);
+            h2 = cmul(h1, h1);
+            h3 = cmul(h2, h1);
+        }
+    }
+    // W^(e0 + e1 m); m is a compile-time constant after unrolling
+    __device__ __forceinline__ float2 operator()(int m) const
+    {
+        const int c = m & 3, d = (m >> 2) & 3, f = m >> 4;
+        float2 r = a;
+        bool unit = !has_a;
+        auto mul = [&](float2 x) {
+            r = unit ? x : cmul(r, x);
+            unit = false;
+        };
+        if (c)
+            mul(c == 1 ? s1 : c == 2 ? s2 : s3);
+        if (d)
+            mul(d == 1 ? q1 : d == 2 ? q2 : q3);
+        if (f)
+            mul(f == 1 ? h1 : f == 2 ? h2 : h3);
+        return r;
+    }
+};
+
 // Full transform.  Every thread of the workgroup must call it (it contains
 // barriers).  lds must hold FftShape::NPAD float2 for this transform.  On
 // entry v[m] = x[t + m*NT]; on exit v[m] = X[t + m*NT] (unnormalised).

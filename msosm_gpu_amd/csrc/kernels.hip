@@ -260,7 +260,8 @@ __global__ __launch_bounds__(fft_threads(LOG2N)) void k_fft_c2c(
 // ---------------------------------------------------------------------------
 // Two-pass (four-step) C2C for N = N1 * N2 > 2^14:
 //   pass A: for every column n2, DFT_N1 over x[N2*n1 + n2], times
-//           W_N^(n2*k1), stored to T[k1*N2 + n2];
+//           W_N^(n2*k1) (FP32 sincospi and digit products), stored to
+//           T[k1*N2 + n2];
 //   pass B: for every row k1, DFT_N2 over T[k1*N2 + n2], stored to
 //           X[k1 + N1*k2].
 // Pass A runs C columns per workgroup with the column index fastest across
@@ -305,14 +306,12 @@ __global__ __launch_bounds__(cols_threads(LOG2N)) void k_fft_passA(
     if (valid) {
         const long long ntot = 1ll << log2ntot;
         float2 *dst = out + (b << log2ntot) + n2;
+        // W_ntot^(n2 k1) for k1 = t + m NT
+        const ColumnTwiddles<E, INV> tw(S::NT > 1 ? n2 * t : 0, n2 * S::NT, ntot);
 #pragma unroll
         for (int m = 0; m < E; m++) {
-            long long k1 = t + m * S::NT;
-            long long e = (n2 * k1) & (ntot - 1);
-            double s, co;
-            sincospi(2.0 * (double)e / (double)ntot, &s, &co);
-            float2 w = make_float2((float)co, INV ? (float)s : (float)(-s));
-            dst[k1 * N2] = cmul(v[m], w);
+            const long long k1 = t + m * S::NT;
+            dst[k1 * N2] = (m == 0 && !tw.has_a) ? v[m] : cmul(v[m], tw(m));
         }
     }
 }
