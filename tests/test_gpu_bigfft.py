@@ -132,3 +132,30 @@ def test_fp16_row_scratch_full_scale(device, kind):
     err = rel_l2(y, oracle_out(bw, dm, f0, M, u))
     print(f"fp16 row scratch {kind}: rel L2 vs FP64 {err:.2e}", flush=True)
     assert err < 1e-3
+
+
+@pytest.mark.parametrize("form,gate", [(ms.FORM_FUSED_F32, 1e-5), (ms.FORM_FUSED_F16, 1e-3)])
+@pytest.mark.parametrize("log2n,count,calls,dm", [(16, 80, 3, 57.3), (15, 300, 2, 30.0)],
+                         ids=["2^16x80", "2^15x300"])
+def test_launches_with_more_rows_than_cus(device, form, gate, log2n, count, calls, dm):
+    """Calls whose forward row launch has more 2^14-point rows than the GPU
+    has CUs (2^16 x 80 blocks: 320 rows, runs of 2; 2^15 x 300 blocks: 600
+    rows, runs of 3): the row pass then runs rows in runs per workgroup.  The
+    last call's last two blocks against the FP64 same-plan oracle, past the
+    delay line."""
+    bw, f0 = 64e6, 1368e6
+    M = 1 << (log2n - 1)
+    p = ms.Plan(bw, dm, f0, fftpoint=2 * M, count=count, form=form)
+    assert p.M == M and not p.clamped
+    K = p.K_max
+    u = make_input(calls * count * M, 4321 + log2n)
+    print(f"rows > CUs: 2^{log2n} count={count} K={K} form={form}", flush=True)
+    y = run_engine(p, u, device=device)[0]
+    b0 = calls * count - 2
+    s0 = max(0, b0 - K - 1)
+    x = oracle.unpack_u16(u[2 * s0 * M:]).astype(np.complex128)
+    d, _, _ = oracle.delay_points(bw, dm, f0, M)
+    ref = oracle.msosm_run(x, M, d, oracle.dedisp_params(bw, dm, f0, M), b0=b0 - s0, nb=2)
+    err = rel_l2(y[b0 * M:], ref)
+    print(f"  rel L2 vs FP64 {err:.2e} (gate {gate:g})", flush=True)
+    assert err < gate

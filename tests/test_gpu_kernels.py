@@ -142,3 +142,37 @@ def test_discard_and_multiply(device):
     torch.cuda.synchronize()
     exp = (x[None].astype(np.complex128) / (2 * M)) * b[:, None, :]
     assert rel_l2(prod.cpu().numpy(), exp) < 1e-6
+
+
+@pytest.mark.parametrize("log2n", [22, 24, 26, 28])
+def test_fft_c2c_large_closed_form(device, log2n):
+    """Two-pass transforms of 2^22 .. 2^28 points (column passes of 2^11 ..
+    2^14 points, the last with 32 points per thread) on two impulses, whose
+    spectrum W^(n0 k) + 0.5i W^(n1 k) is known in closed form: 2^16 sampled
+    bins against it in FP64 (phases from integer arithmetic mod N), both
+    directions."""
+    import torch
+    n = 1 << log2n
+    n0, n1 = (n // 3) | 1, n - 12345
+    x = torch.zeros(n, dtype=torch.complex64, device=device)
+    x[n0] = 1.0
+    x[n1] = 0.5j
+    out = torch.empty_like(x)
+    rng = np.random.default_rng(log2n)
+    ks = np.unique(np.concatenate([rng.integers(0, n, 1 << 16), [0, 1, n // 2, n - 1]]))
+    kd = torch.from_numpy(ks).to(device)
+    for direction in (-1, +1):
+        print(f"fft n=2^{log2n} direction={direction}", flush=True)
+        ms.fft_c2c(x, out, n, 1, direction=direction)
+        torch.cuda.synchronize()
+        got = out[kd].cpu().numpy().astype(np.complex128)
+        ph0 = (n0 * ks.astype(object)) % n
+        ph1 = (n1 * ks.astype(object)) % n
+        e0 = np.exp(direction * 2j * np.pi * ph0.astype(np.float64) / n)
+        e1 = np.exp(direction * 2j * np.pi * ph1.astype(np.float64) / n)
+        ref = e0 + 0.5j * e1
+        err = rel_l2(got, ref)
+        print(f"  rel L2 on {ks.size} bins {err:.2e}", flush=True)
+        assert err < 2e-6
+    del x, out
+    torch.cuda.empty_cache()
