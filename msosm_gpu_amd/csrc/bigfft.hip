@@ -205,7 +205,7 @@ __global__ __launch_bounds__(RowS::NT) void k_big_fwd_rows_nb(const void *__rest
 #pragma unroll
     for (int m = 0; m < E; m++)
         raw[m] = src[(long long)r0 * kN2 + t + m * NT];
-    __builtin_amdgcn_s_waitcnt(0x0F70); // vmcnt(0): the loop's first iteration has nothing in flight
+    __builtin_amdgcn_s_waitcnt(kWaitVmcnt0); // the loop's first iteration has nothing in flight
     for (int i = 0; i < nr; i++) {
         const int r = r0 + i;
         float2 v[E];
@@ -360,21 +360,6 @@ __global__ void k_big_permute(const float2 *__restrict__ src, void *__restrict__
 
 } // namespace
 
-// Rows per workgroup of the 2^14-point row transforms: one while a launch's
-// rows fit in one round of workgroups (one such workgroup per CU: its LDS),
-// else runs that spread the rows over the CUs once.
-static int rows_per_wg(int nrows)
-{
-    static const int cus = [] {
-        int dev = 0, n = 0;
-        if (hipGetDevice(&dev) != hipSuccess ||
-            hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0)
-            n = 256;
-        return n;
-    }();
-    return nrows <= cus ? 1 : (nrows + cus - 1) / cus;
-}
-
 bool big_supported(int log2n) { return log2n > kRowLog2 && log2n <= kRowLog2 + 7; }
 
 long long big_position_bin(int log2n, long long p)
@@ -425,7 +410,7 @@ hipError_t launch_big_forward(const uint32_t *cur, const uint32_t *tail, uint32_
         if (e != hipSuccess)
             return e;
         const int nrows = nw << l1;
-        const int nrpw = rows_per_wg(nrows);
+        const int nrpw = runs_per_wg(nrows);
         if (nrpw > 1) {
             const unsigned grid = (unsigned)((nrows + nrpw - 1) / nrpw);
             if (f16)

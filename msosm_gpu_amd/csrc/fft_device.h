@@ -22,6 +22,11 @@ namespace msosm {
 
 __host__ __device__ constexpr int ilog2c(int v) { return v <= 1 ? 0 : 1 + ilog2c(v >> 1); }
 
+// s_waitcnt vmcnt(0) as an immediate (gfx9 encoding: vmcnt in bits 3:0 and
+// 15:14, expcnt in 6:4, lgkmcnt in 11:8; the counters at their maxima are not
+// waited for)
+constexpr int kWaitVmcnt0 = 0x0F70;
+
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
 __device__ __forceinline__ float2 cmul(float2 a, float2 b)

@@ -33,6 +33,13 @@ hipError_t launch_fwd_u16(const uint32_t *cur, const uint32_t *tail, uint32_t *t
 // of one trial per workgroup: the largest power of two up to max_nbpw that
 // leaves at least 512 workgroups for rows x count blocks, else 1.
 int blocks_per_wg(int count, long long rows, int max_nbpw);
+// CUs of the calling thread's current device (cached per device; 256 if the
+// query fails)
+int device_cu_count();
+// Transforms per workgroup for a launch of `items` 2^14-point transforms run
+// one workgroup per CU at a time (their LDS): one while they fit in one round
+// of workgroups, else runs that spread them over the CUs once.
+int runs_per_wg(long long items);
 // nbpw: blocks per workgroup for sizes with one transform per workgroup
 // (0 = blocks_per_wg(count, ndm, 32)).
 hipError_t launch_inv_fused(const void *ring, const void *H, const uint16_t *delay, const uint32_t *dpk, float2 *out,

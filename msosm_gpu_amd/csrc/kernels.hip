@@ -14,6 +14,7 @@
 #include <hip/hip_fp16.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstdlib>
 
 #include "elems.h"
@@ -474,7 +475,7 @@ __global__ __launch_bounds__((1 << 14) / 32) void k_fwd_u16_nb(
     // body has no branches for the same reason; the last window of a run
     // loads itself again, which leaves the call's last window in u for the
     // tail.
-    __builtin_amdgcn_s_waitcnt(0x0F70); // vmcnt(0)
+    __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
     for (int i = 0; i < nw; i++) {
         const int w = w0 + i;
         float2 v[E];
@@ -790,19 +791,27 @@ hipError_t launch_fft_c2c(const float2 *in, float2 *out, int log2n, long long ba
 
 bool fused_supported(int log2n) { return log2n >= 2 && log2n <= kMaxLog2SinglePass; }
 
-// Windows per workgroup of the 2^14-point forward transform: one while the
-// call's windows fit in one round of workgroups (one such workgroup per CU:
-// its LDS), else runs that spread the windows over the CUs once.
-static int fwd_windows_per_wg(int count)
+int device_cu_count()
 {
-    static const int cus = [] {
-        int dev = 0, n = 0;
-        if (hipGetDevice(&dev) != hipSuccess ||
-            hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0)
-            n = 256;
+    constexpr int kMaxDevices = 64;
+    static std::atomic<int> cache[kMaxDevices];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0)
+        return 256;
+    int n = dev < kMaxDevices ? cache[dev].load(std::memory_order_relaxed) : 0;
+    if (n > 0)
         return n;
-    }();
-    return count <= cus ? 1 : (count + cus - 1) / cus;
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0)
+        n = 256;
+    if (dev < kMaxDevices)
+        cache[dev].store(n, std::memory_order_relaxed);
+    return n;
+}
+
+int runs_per_wg(long long items)
+{
+    const long long cus = device_cu_count();
+    return items <= cus ? 1 : (int)((items + cus - 1) / cus);
 }
 
 template <int LOG2N>
@@ -827,7 +836,7 @@ static hipError_t fwd_u16(const uint32_t *cur, const uint32_t *tail, uint32_t *t
     if constexpr (LOG2N == 14) {
         // more windows than run at once: runs of windows per workgroup (the
         // kernel reloads the call's last window, which needs count >= 2)
-        const int nwpw = fwd_windows_per_wg(count);
+        const int nwpw = runs_per_wg(count);
         if (nwpw > 1 && count >= 2) {
             const int g = (count + nwpw - 1) / nwpw;
             if (f16)
