@@ -362,6 +362,9 @@ int create_engine(const msosm::Plan &plan, void *stream, msosm_engine **out)
                               "mixed-precision gate; use MSOSM_FORM_HANKEL_BF16X3 (hi+lo split) or "
                               "MSOSM_FORM_HANKEL_F16"));
     e->plan_hash = plan_identity(plan);
+    // the launchers size their runs from the device's CU count, cached per
+    // device on first use: fill the cache here, outside any graph capture
+    (void)msosm::device_cu_count();
     e->big = (e->form == MSOSM_FORM_FUSED_F32 || e->form == MSOSM_FORM_FUSED_F16) && msosm::big_supported(e->log2n);
     if (e->form != MSOSM_FORM_PLAIN_F32 && !msosm::fused_supported(e->log2n) && !e->big)
         return bail(set_error(MSOSM_ERR_UNSUPPORTED, "the fused forms support FFT sizes 4..2097152, the matrix-core "
