@@ -261,7 +261,11 @@ __global__ __launch_bounds__(RowS::NT) void k_big_inv_rows(const void *__restric
         step[t] = twiddle_f64(((long long)k1 * NT * t) & (N - 1), N, true);
     const float2 base = twiddle_f64(((long long)k1 * t) & (N - 1), N, true);
     const long long p0 = (long long)k1 * kN2;
-    const TT *rg = reinterpret_cast<const TT *>(ring);
+    // the ring's row k1 of slot sl starts at byte (sl N + p0) sizeof(TT): the
+    // engine keeps fused rings below 4 GiB, so 32-bit offsets from a scalar
+    // base, with N = 2^(l1 + 14) a shift
+    const char *rgB = reinterpret_cast<const char *>(ring) + p0 * (long long)sizeof(TT);
+    const int lshift = l1 + kRowLog2;
     const TT *hrow = reinterpret_cast<const TT *>(Hp) + (long long)dm * N + p0;
     const uint16_t *drow = dp + (long long)dm * N + p0;
     const int s0 = (int)((g0 + b) % rx);
@@ -271,7 +275,8 @@ __global__ __launch_bounds__(RowS::NT) void k_big_inv_rows(const void *__restric
         const int q = t + m * NT;
         int sl = s0 - (int)drow[q];
         sl += sl < 0 ? rx : 0;
-        v[m] = R::mul(rg[(long long)sl * N + p0 + q], hrow[q]);
+        const uint32_t off = (((uint32_t)sl << lshift) + (uint32_t)q) * (uint32_t)sizeof(TT);
+        v[m] = R::mul(*reinterpret_cast<const TT *>(rgB + off), hrow[q]);
     }
     fft_block<kRowLog2, E, true, true>(v, smem, twlo, twhi, t, twrows); // barriers: step[] is visible after it
     const long long roff = ((long long)ul * N1 + k1) * kN2;
