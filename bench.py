@@ -213,6 +213,8 @@ def main():
                     alg_bytes_per_launch=alg_bytes, ms_per_launch=st["ms_per_launch"])
     kernels = kernel_rooflines(results, plan_info, len(dms), fold, hankel_flops, counters)
     copy_gbs = copy_bandwidth(torch, dev)
+    fftcmp = fft_vs_library(ms, torch, dev)
+    log(f"batched inverse FFT vs torch.fft (ROCm's library): {json.dumps(fftcmp)}")
     log(f"device copy (1 GiB read + 1 GiB write): {copy_gbs:.0f} GB/s")
     kernels[kname] = {k: roofline[k] for k in ("achieved", "unit", "peak", "frac", "ms_per_launch")}
     if head_traffic:
@@ -256,6 +258,7 @@ def main():
                      "calibration": ({k: head_traffic[k] for k in ("fetch_factor", "write_factor")}
                                      if head_traffic else None)},
         "hbm_copy_gbs": round(copy_gbs, 1),
+        "fft_vs_rocm_library": fftcmp,
         "host_input_pcie": pcie,
         "rel_l2_vs_fp64": fp64 or None,
         "rel_l2_vs_fp64_osm": osm,
@@ -587,6 +590,44 @@ def copy_bandwidth(torch, dev, reps=5):
         best = max(best, 2 * src.numel() * 4 / (e0.elapsed_time(e1) / 1e3) / 1e9)
     del src, dst
     return best
+
+
+def fft_vs_library(ms, torch, dev, shapes=((14, 16384), (16, 1024)), reps=10):
+    """The build's batched C2C inverse (msosm_k_fft_c2c) beside ROCm's FFT
+    library as torch.fft.ifft exposes it (a labelled comparator, not the
+    product path), same complex64 input: time per batch and the two outputs'
+    relative difference.  16384 x 2^14 is one call of the headline workload's
+    inverse transforms (256 trials x 64 blocks)."""
+    res = {}
+    for log2n, batch in shapes:
+        n = 1 << log2n
+        g = torch.Generator(device=dev).manual_seed(log2n)
+        x = torch.randn(batch, n, dtype=torch.complex64, device=dev, generator=g)
+        out = torch.empty_like(x)
+
+        def timed(fn):
+            for _ in range(2):
+                fn()
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(reps):
+                fn()
+            e1.record()
+            torch.cuda.synchronize()
+            return e0.elapsed_time(e1) / reps
+
+        ours = timed(lambda: ms.fft_c2c(x, out, n, batch, direction=+1))
+        holder = {}
+        lib = timed(lambda: holder.__setitem__("y", torch.fft.ifft(x, dim=1)))
+        diff = (torch.linalg.norm(out / n - holder["y"]) / torch.linalg.norm(holder["y"])).item()
+        gb = batch * n * 16 / 1e9
+        res[f"2^{log2n} x {batch}"] = dict(ours_ms=round(ours, 4), torch_fft_ms=round(lib, 4),
+                                           ours_gbs=round(gb / ours * 1e3, 1), torch_fft_gbs=round(gb / lib * 1e3, 1),
+                                           rel_diff=float(f"{diff:.3g}"))
+        del x, out, holder
+    torch.cuda.empty_cache()
+    return res
 
 
 def host_input_rate(torch, eng, u, n, ncalls, ndm, steps):
