@@ -11,12 +11,15 @@
 // stores (discard, scale, FP16 packing) around fft_block().
 //
 // Twiddles W_N^e = exp(-2 pi i e/N) come from two small LDS tables
-// (W = hi[e >> LS] * lo[e & (2^LS-1)]) filled once per workgroup in FP64.
+// (W = hi[e >> LS] * lo[e & (2^LS-1)]) copied once per workgroup from
+// read-only tables evaluated at compile time (twiddle_tables.h).
 // LDS addresses are padded by one element every E elements so the strided
 // Stockham writes do not all land in the same banks.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <hip/hip_fp16.h>
+
+#include "twiddle_tables.h"
 
 namespace msosm {
 
@@ -187,26 +190,29 @@ struct FftShape {
 template <int LOGE>
 __device__ __forceinline__ int lidx(int i) { return i + (i >> LOGE); }
 
-// Fill the two twiddle tables (forward sign) for a transform of 2^LOG2N.
+// The twiddle tables in device memory (read-only, forward sign): the
+// two-level table of each transform size and the middle pass's row table.
+template <int LOG2N>
+__device__ constexpr twc::TwTable<LOG2N> kTwTable = twc::make_twtable<LOG2N>();
+__device__ constexpr twc::TwRows kTwRowsTable = twc::make_twrows();
+
+// Copy the two twiddle tables (forward sign) of a transform of 2^LOG2N into
+// LDS.  The thread index passes through an empty asm so that the copy's index
+// arithmetic is not shared with the caller's loop: otherwise the compiler
+// keeps fewer of the caller's loop-invariant offsets in registers (measured:
+// 7 more vector instructions per block in the 2^14-point fused inverse).
 template <int LOG2N, int E>
 __device__ __forceinline__ void fill_twiddles(float2 *twlo, float2 *twhi, int tid, int nthreads)
 {
     using S = FftShape<LOG2N, E>;
-    for (int i = tid; i < S::TWLO + S::TWHI; i += nthreads) {
-        double frac;
-        float2 *dst;
-        if (i < S::TWLO) {
-            frac = (double)i / (double)S::N;
-            dst = twlo + i;
-        } else {
-            int k = i - S::TWLO;
-            frac = (double)k * (double)S::TWLO / (double)S::N;
-            dst = twhi + k;
-        }
-        double s, c;
-        sincospi(2.0 * frac, &s, &c);
-        *dst = make_float2((float)c, (float)(-s));
-    }
+    using TT = twc::TwTable<LOG2N>;
+    static_assert(TT::LO == S::TWLO && TT::HI == S::TWHI, "table shape");
+    const float2 *src = reinterpret_cast<const float2 *>(kTwTable<LOG2N>.v);
+    asm volatile("" : "+v"(tid));
+    for (int i = tid; i < S::TWLO; i += nthreads)
+        twlo[i] = src[i];
+    for (int i = tid; i < S::TWHI; i += nthreads)
+        twhi[i] = src[S::TWLO + i];
 }
 
 // Row table of the inter-pass twiddles of a pass with NS = 32 and radix 32
@@ -214,17 +220,15 @@ __device__ __forceinline__ void fill_twiddles(float2 *twlo, float2 *twhi, int ti
 // row k holds W_1024^(k r) for r < 32 (forward sign), rows padded to 33
 // entries.  A thread's k is fixed, so its 31 reads are one base address plus
 // immediate offsets, and the 32 rows a wave reads start in distinct banks.
-constexpr int kTwRowN = 32;      // rows (NS) and columns (R)
-constexpr int kTwRowStride = 33; // padded row length
-constexpr int kTwRowElems = kTwRowN * kTwRowStride;
+using twc::kTwRowElems;
+using twc::kTwRowN;
+using twc::kTwRowStride;
 __device__ __forceinline__ void fill_twrows(float2 *tw, int tid, int nthreads)
 {
-    for (int i = tid; i < kTwRowElems; i += nthreads) {
-        const int k = i / kTwRowStride, r = i - k * kTwRowStride;
-        double s, c;
-        sincospi(2.0 * (double)((k * r) & (kTwRowN * kTwRowN - 1)) / (double)(kTwRowN * kTwRowN), &s, &c);
-        tw[i] = make_float2((float)c, (float)(-s));
-    }
+    const float2 *src = reinterpret_cast<const float2 *>(kTwRowsTable.v);
+    asm volatile("" : "+v"(tid));
+    for (int i = tid; i < kTwRowElems; i += nthreads)
+        tw[i] = src[i];
 }
 
 template <int LOG2N, int E, bool INV>

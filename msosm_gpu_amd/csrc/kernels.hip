@@ -630,9 +630,17 @@ __global__ __launch_bounds__((1 << LOG2N) / E) void k_inv_fused_nb(
                 const int d = q ? (int)(pair >> 16) : (int)(pair & 0xFFFFu);
                 int sl = slot - d;
                 sl += sl < 0 ? rx : 0;
-                const uint32_t off =
-                    ((uint32_t)sl * (uint32_t)N + (uint32_t)(t + m * NT)) * (uint32_t)sizeof(T);
-                raw[m] = *reinterpret_cast<const T *>(ringB + off);
+                if constexpr (F16) {
+                    const uint32_t off =
+                        ((uint32_t)sl * (uint32_t)N + (uint32_t)(t + m * NT)) * (uint32_t)sizeof(T);
+                    raw[m] = *reinterpret_cast<const T *>(ringB + off);
+                } else {
+                    // the point's column offset in the scalar base (one
+                    // instruction per address; measured faster for FP32 rings
+                    // only)
+                    const uint32_t off = ((uint32_t)sl * (uint32_t)N + (uint32_t)t) * (uint32_t)sizeof(T);
+                    raw[m] = *reinterpret_cast<const T *>(ringB + (long long)m * NT * (long long)sizeof(T) + off);
+                }
             }
         }
     };
