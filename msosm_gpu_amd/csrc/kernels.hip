@@ -434,7 +434,7 @@ __global__ __launch_bounds__(fft_threads(LOG2N)) void k_fwd_u16(
 
 // Fused forward for calls with many 2^14-point windows (more than the GPU
 // runs at once): each workgroup transforms a contiguous run of nwpw windows,
-// so the twiddle tables are built once per run, the next window's codes are
+// so the twiddle tables are copied into LDS once per run, the next window's codes are
 // loaded into registers while the current window is transformed, and the
 // half of each window that the previous one also read is an L2 hit on the
 // same XCD.  The middle pass reads its twiddles from the row table.  Same
