@@ -62,6 +62,9 @@ def parse():
                     help="short run for hardware-counter passes: a few calls of each --pmc-forms form, phase "
                          "folds of the last output and a 1 GiB device copy (the counter calibration), no timing")
     ap.add_argument("--pmc-forms", default="mixed_f16", help="comma list of forms the counter run drives")
+    ap.add_argument("--pmc-single128", action="store_true",
+                    help="with --pmc-workload: drive the 128 MHz single-trial workload instead (DM 100, f0 1336 MHz, "
+                         "FFT --fftpoint, --count blocks per call) to count its kernels")
     ap.add_argument("--no-counters", action="store_true",
                     help="skip the rocprofv3 counter passes (traffic and matrix-core counters are then null)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -97,6 +100,8 @@ def main():
     bw, f0 = 64e6, 1368e6
     dms = np.linspace(0.0, 100.0, args.num_dms).astype(np.float32)
     if args.pmc_workload:
+        if args.pmc_single128:
+            bw, f0, dms = 128e6, 1336e6, np.array([100.0], np.float32)
         pmc_workload(ms, torch, dev, bw, f0, dms, args)
         return
     if args.only_config:
