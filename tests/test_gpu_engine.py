@@ -218,15 +218,19 @@ def test_forms_agree(device):
         assert rel_l2(outs[ms.FORM_FUSED_F16][i], outs[ms.FORM_PLAIN_F32][i]) < MIXED_TOL
 
 
-def test_osm_two_pass_fft(device):
-    # OSM plan of the 16 MHz / DM 10 case: M = 16384, FFT 32768 (two-pass)
+@pytest.mark.parametrize("form", [ms.FORM_PLAIN_F32, ms.FORM_FUSED_F32, ms.FORM_FUSED_F16])
+def test_osm_two_pass_fft(device, form):
+    # OSM plan of the 16 MHz / DM 10 case: M = 16384, FFT 32768 (two-pass for
+    # the plain form, four-step with a one-slot delay line for the fused ones)
     bw, dm, f0 = 16e6, 10.0, 1392e6
-    p = ms.Plan(bw, dm, f0, fftpoint=0, count=2, method=ms.METHOD_OSM)
+    p = ms.Plan(bw, dm, f0, fftpoint=0, count=2, method=ms.METHOD_OSM, form=form)
     assert p.M == 16384 and p.K_max == 1
     u = make_input(4 * 16384, 21)
     y = run_engine(p, u, device=device)[0]
     ref = oracle_out(bw, dm, f0, 16384, u, method=ms.METHOD_OSM)
-    assert rel_l2(y, ref) < FP32_TOL
+    err = rel_l2(y, ref)
+    print(f"OSM FFT 32768 form {form}: rel L2 {err:.2e}", flush=True)
+    assert err < TOL[form]
 
 
 @pytest.mark.parametrize("form", [ms.FORM_FUSED_F32, ms.FORM_FUSED_F16, ms.FORM_HANKEL_F16, ms.FORM_HANKEL_BF16X3])
