@@ -15,6 +15,16 @@ __device__ __forceinline__ float2 unpack_u16pair(uint32_t u)
     float fy = (float)y - (y != 0 ? 32768.0f : 0.0f);
     return make_float2(fx, fy);
 }
+// The same scaled by a power of two s (exact: (x - 32768) s as one FMA of
+// exact terms), so that a transform of the scaled samples equals the scaled
+// transform bit for bit and the scale costs nothing at the store.
+__device__ __forceinline__ float2 unpack_u16pair_scaled(uint32_t u, float s)
+{
+    uint32_t x = u & 0xFFFFu, y = u >> 16;
+    float fx = __builtin_fmaf((float)x, s, x != 0 ? -32768.0f * s : 0.0f);
+    float fy = __builtin_fmaf((float)y, s, y != 0 ? -32768.0f * s : 0.0f);
+    return make_float2(fx, fy);
+}
 
 template <bool F16>
 struct RingElem;

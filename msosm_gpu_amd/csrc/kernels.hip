@@ -406,22 +406,24 @@ __global__ __launch_bounds__(fft_threads(LOG2N)) void k_fwd_u16(
         for (int m = 0; m < E; m++)
             u[m] = 0;
     }
+    // the FP16 ring holds X/N: the samples are scaled by 1/N (a power of two)
+    // on unpacking, which gives the same values as scaling the transform
+    constexpr float invN = 1.0f / (float)N;
     float2 v[E];
 #pragma unroll
     for (int m = 0; m < E; m++)
-        v[m] = unpack_u16pair(u[m]);
+        v[m] = F16 ? unpack_u16pair_scaled(u[m], invN) : unpack_u16pair(u[m]);
     fft_block<LOG2N, E, false>(v, smem + tl * S::NPAD, twlo, twhi, t);
     if (valid) {
         const long long slot = (g0 + w) % rx;
         if constexpr (F16) {
             __half2 *dst = reinterpret_cast<__half2 *>(ring) + slot * N;
-            constexpr float invN = 1.0f / (float)N;
 #pragma unroll
             for (int m = 0; m < E; m++) {
                 // MFMA_ORDER: bin n at position mfft::ring_pos(n), the gather
                 // order of the matrix-core inverse (2^14 points)
                 const int n = t + m * S::NT;
-                dst[MFMA_ORDER ? mfft::ring_pos(n) : n] = __floats2half2_rn(v[m].x * invN, v[m].y * invN);
+                dst[MFMA_ORDER ? mfft::ring_pos(n) : n] = __floats2half2_rn(v[m].x, v[m].y);
             }
         } else {
             float2 *dst = reinterpret_cast<float2 *>(ring) + slot * N;
@@ -478,10 +480,12 @@ __global__ __launch_bounds__((1 << 14) / 32) void k_fwd_u16_nb(
     __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
     for (int i = 0; i < nw; i++) {
         const int w = w0 + i;
+        // FP16 ring: X/N from samples scaled by 1/N on unpacking (exact)
+        constexpr float invN = 1.0f / (float)N;
         float2 v[E];
 #pragma unroll
         for (int m = 0; m < E; m++)
-            v[m] = unpack_u16pair(u[m]);
+            v[m] = F16 ? unpack_u16pair_scaled(u[m], invN) : unpack_u16pair(u[m]);
         // window min(w + 1, count - 1) starts at (min(w + 1, count - 1) - 1) M;
         // with count >= 2 that is in [0, (count - 2) M] and the window ends by
         // count M, inside this call's input
@@ -493,10 +497,9 @@ __global__ __launch_bounds__((1 << 14) / 32) void k_fwd_u16_nb(
         const long long slot = (g0 + w) % rx;
         if constexpr (F16) {
             __half2 *dst = reinterpret_cast<__half2 *>(ring) + slot * N;
-            constexpr float invN = 1.0f / (float)N;
 #pragma unroll
             for (int m = 0; m < E; m++)
-                dst[t + m * NT] = __floats2half2_rn(v[m].x * invN, v[m].y * invN);
+                dst[t + m * NT] = __floats2half2_rn(v[m].x, v[m].y);
         } else {
             float2 *dst = reinterpret_cast<float2 *>(ring) + slot * N;
 #pragma unroll
