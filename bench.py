@@ -73,7 +73,7 @@ def parse():
     ap.add_argument("--cpu-log2-samples", type=int, default=23)
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the short runs of the other workloads (single trial at 16 and 128 MHz, 24 streams)")
-    ap.add_argument("--only-config", choices=("single16", "single128", "streams24"),
+    ap.add_argument("--only-config", choices=("single16", "single128", "streams24", "science"),
                     help="run only this one of the other workloads and print its results as one JSON line (no "
                          "headline; for profiling a workload on its own)")
     return ap.parse_args()
@@ -312,11 +312,12 @@ def time_engine(ms, torch, plan, d_in, L, steps=2, warmup=1, gate=None):
                samples=calls * n, fftpoint=2 * plan.M, count=plan.count)
     rows = None
     if gate is not None:
+        nb = min(2, plan.count)
         b0 = (calls - 1) * plan.count
-        rows = eng.output_c64()[:, :2].reshape(plan.num_dms, -1).astype(np.complex128)
-        ref = fp64_same_plan(eng, plan, device_reader(d_in), (b0, b0 + 2))
+        rows = eng.output_c64()[:, :nb].reshape(plan.num_dms, -1).astype(np.complex128)
+        ref = fp64_same_plan(eng, plan, device_reader(d_in), (b0, b0 + nb))
         err = max(rel(rows[i], ref[i]) for i in range(plan.num_dms))
-        res["rel_l2_vs_fp64"] = dict(max=err, blocks=[b0, b0 + 2], gate=gate, passed=bool(err < gate))
+        res["rel_l2_vs_fp64"] = dict(max=err, blocks=[b0, b0 + nb], gate=gate, passed=bool(err < gate))
         res["_check"] = (b0 * plan.M, rows, ref)
     eng.close()
     return res
@@ -341,8 +342,9 @@ def other_configs(ms, torch, dev, only=None):
     (DM 100 at its plan, FFT 2^16 with K 152, and at FFT 2^14 with K 608), and
     24 independent streams of 8-64 MHz, 2^20-2^25 samples, 1-64 trials,
     autotuned FFT sizes, forms and launch shapes, up to 2^22 samples per call,
-    each phase-folded, replayed as one graph.  only = "single16",
-    "single128" or "streams24" runs that one alone."""
+    each phase-folded, replayed as one graph, and the reference's science
+    plan (FFT 2^21, one block per call).  only = "single16", "single128",
+    "streams24" or "science" runs that one alone."""
     out = {}
     if only in (None, "single16"):
         out.update(config_single16(ms, torch, dev))
@@ -350,7 +352,27 @@ def other_configs(ms, torch, dev, only=None):
         out.update(config_single128(ms, torch, dev))
     if only in (None, "streams24"):
         out.update(config_streams24(ms, torch, dev))
+    if only in (None, "science"):
+        out.update(config_science(ms, torch, dev))
     return out
+
+
+def config_science(ms, torch, dev):
+    """The plan the reference's science driver uses (msosm_psrdada.cpp:94-95:
+    fftpoint 2097152, count 1) on a single 64 MHz trial (DM 100, f0 1368 MHz):
+    2^26 samples, 64 calls of one block of 2^20 output samples."""
+    L = 1 << 26
+    d_in = random_u16_words(torch, dev, L, 0x5EED0005)
+    c = {}
+    for name, form, gate in (("mixed_f16", ms.FORM_FUSED_F16, 1e-3), ("fused_f32", ms.FORM_FUSED_F32, 1e-5),
+                             ("plain_f32", ms.FORM_PLAIN_F32, 1e-5)):
+        c[name] = time_engine(ms, torch, ms.Plan(64e6, 100.0, 1368e6, fftpoint=1 << 21, count=1, form=form), d_in,
+                              L, steps=2, gate=gate)
+        c[name].pop("_check", None)
+    del d_in
+    torch.cuda.empty_cache()
+    return {"single trial, 64 MHz, DM 100, 2^26 samples, FFT 2^21, 1 block per call (the reference's science "
+            "plan)": c}
 
 
 def config_single16(ms, torch, dev):
@@ -803,8 +825,8 @@ def fp64_same_plan(eng, plan, read, blocks):
     d = np.stack([plan.delay_points(i) for i in range(ndm)]).astype(np.int64)
     H = np.stack([eng.dedisp_params(i) for i in range(ndm)]).astype(np.complex128)
     g0 = max(0, b0 - int(d.max()))
-    # window of block g: samples [(g-1)M, (g+1)M)
-    x = unpacked(read, (g0 - 1) * M, (b1 + 1) * M)
+    # window of block g: samples [(g-1)M, (g+1)M); the last block b1 - 1 ends at b1 M
+    x = unpacked(read, (g0 - 1) * M, b1 * M)
     win = np.lib.stride_tricks.sliding_window_view(x, N)[::M][:b1 - g0]
     X = np.fft.fft(win, axis=1)
     n = np.arange(N)
