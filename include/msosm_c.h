@@ -282,7 +282,7 @@ int msosm_fold_intensity_ptr(msosm_fold *f, void **dev_ptr);
  * fold_gpu.h:31, which fold_data reads). */
 int msosm_fold_load_intensity(msosm_fold *f, const float *src, int where);
 
-/* Phase fold of engine output (north-star fold; no reference counterpart:
+/* Phase fold of engine output (no reference counterpart:
  * the reference folds on the CPU with integer periods, fold_gpu.cpp:48-173).
  * For each of ndm trials (rows of `stride` complex64 samples at a, and at b
  * for a second polarisation or NULL), sample i with global index n0 + i goes
@@ -294,8 +294,8 @@ int msosm_fold_load_intensity(msosm_fold *f, const float *src, int where);
 int msosm_fold_phase(const void *a, const void *b, size_t n, int ndm, size_t stride, long long n0,
                      double period_samples, int nbins, int power, float *prof, float *hits, void *stream);
 
-/* Uneven multi-stream batch in one graph launch (north-star component; no
- * reference counterpart: the reference runs one CUDA stream per engine with
+/* Uneven multi-stream batch in one graph launch (no reference
+ * counterpart: the reference runs one CUDA stream per engine with
  * host launches per block, msosm_stream/msosm_gpu_batch_stream.cpp:105-149).
  * engines[i] (distinct, created by the caller, timing off) each filter their
  * own resident device input inputs[i] of num_samples[i] packed uint16 pairs
@@ -340,7 +340,7 @@ int msosm_autotune_fftpoint(const msosm_plan_desc *desc, unsigned long min_fftpo
                             unsigned long target_samples, int calls, void *stream, unsigned long *best_fftpoint,
                             double *best_ns_per_dm_sample, int *best_form);
 
-/* ---- learned DM proposal (north-star component; no reference counterpart:
+/* ---- learned DM proposal (no reference counterpart:
  * the reference dedisperses every trial of a grid,
  * test_processing_speed_msosm_dmtrial.cpp) -------------------------------
  * Device pointers throughout; activations are FP32 [B][C][T], convolution
