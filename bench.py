@@ -87,7 +87,7 @@ def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the engine shards over DM trials with no exchange beyond an input "
-              "broadcast, and this build runs on single-GPU boxes only; run with --gpus 1", file=sys.stderr)
+              "broadcast; this benchmark runs on one GPU only; run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     import torch
 

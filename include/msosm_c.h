@@ -31,7 +31,7 @@ extern "C" {
 #define MSOSM_ERR_ARG 1         /* invalid argument                          */
 #define MSOSM_ERR_HIP 2         /* a HIP runtime call failed                 */
 #define MSOSM_ERR_STATE 3       /* call out of order                         */
-#define MSOSM_ERR_UNSUPPORTED 4 /* valid request this build does not support */
+#define MSOSM_ERR_UNSUPPORTED 4 /* valid request the library does not support */
 #define MSOSM_ERR_ALLOC 5       /* device or host allocation failed          */
 
 /* ---- plan descriptor ------------------------------------------------------ */
@@ -355,7 +355,7 @@ int msosm_sim_disperse(void *X_c64, size_t n, double fs, double f0, double dm, i
 int msosm_sim_quantize(const void *x_c64, size_t n, float x_scale, unsigned int seed, float scale, void *u16_out,
                        void *stream);
 /* Power spectrogram out[nchan][T], T = n / (nchan navg): navg consecutive
- * nchan-point spectra per bin (this build: nchan 256, navg 16), channel 0 =
+ * nchan-point spectra per bin (the library supports nchan 256, navg 16), channel 0 =
  * lowest frequency. */
 int msosm_spectrogram_u16(const void *u16, size_t n, int nchan, int navg, float *out, void *stream);
 int msosm_rows_standardize(float *x, int rows, int T, void *stream);

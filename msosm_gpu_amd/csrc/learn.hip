@@ -626,7 +626,7 @@ int msosm_sim_quantize(const void *x_c64, size_t n, float x_scale, unsigned int 
 int msosm_spectrogram_u16(const void *u16, size_t n, int nchan, int navg, float *out, void *stream)
 {
     if (!u16 || !out || nchan != 256 || navg != 16 || n < (size_t)nchan * navg)
-        return msosm::set_error(MSOSM_ERR_UNSUPPORTED, "spectrogram: this build supports 256 channels x 16 spectra");
+        return msosm::set_error(MSOSM_ERR_UNSUPPORTED, "spectrogram: the library supports 256 channels x 16 spectra");
     const int T = (int)(n / ((size_t)nchan * navg));
     hipLaunchKernelGGL(k_spectrogram256, dim3(T), dim3(256), 0, (hipStream_t)stream, (const uint32_t *)u16, T, out);
     LHIP(hipGetLastError());

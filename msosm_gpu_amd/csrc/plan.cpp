@@ -179,7 +179,7 @@ int build_plan(const msosm_plan_desc &d, Plan &p, std::string &msg)
         }
     }
     if (M < 2 || !is_pow2(M)) {
-        snprintf(buf, sizeof buf, "plan: FFT size 2M = %lu is not supported (this build needs a power of "
+        snprintf(buf, sizeof buf, "plan: FFT size 2M = %lu is not supported (the library needs a power of "
                                   "two >= 4; DM = 0 everywhere needs an explicit fftpoint)", 2 * M);
         msg = buf;
         return MSOSM_ERR_UNSUPPORTED;

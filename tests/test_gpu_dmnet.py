@@ -139,7 +139,7 @@ def test_trained_network_gain(device, capsys):
     incoherently dedispersed spectrogram folded at the period, and scores
     every trial.  For 64 held-out pulsars its top 16 trials are coherently
     dedispersed and folded and their best S/N is compared with the full
-    1024-trial grid's best (the judge, fused FP32 form), against 16 evenly
+    1024-trial grid's best (the reference score: fused FP32 form), against 16 evenly
     spread trials and an untrained network; with a fallback to the full grid
     when no proposed trial reaches S/N 8."""
     import time
